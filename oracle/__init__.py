@@ -1,0 +1,161 @@
+"""CPU oracle for the NUFFT hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker (or the timed CPU baseline).  The product package never imports it.
+
+Two layers:
+
+* ``type1`` / ``type2`` / ``type3`` (+ ``type1_at``): FP64 direct sums in C
+  (``direct.c``), the closed forms of the reference tests
+  (tests/ops_test.py:42-45, :77-84, :112-116).
+* ``finufft_cpu``: the FINUFFT method restated on the CPU (``finufft_cpu.c``
+  plus ``scipy.fft``), used as a stage-level checker and as the CPU baseline.
+
+Axis convention here is the *user* (JAX) one: ``points[d]`` pairs with mode
+axis ``d`` of a C-ordered mode array, exactly as in the reference tests.  The C
+layer uses the native (reversed, x-fastest) order that the FFI receives
+(src/jax_finufft/lowering.py:266-275); the reversal happens in this file.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def lib():
+    """Load ``liboracle.so`` (built by ``make -C oracle`` / ``__graft_entry__.build``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(
+                f"oracle library missing at {_LIB_PATH}; build it with `make -C oracle`"
+            )
+        L = ctypes.CDLL(_LIB_PATH)
+        i64, i32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        L.orc_type1.argtypes = [i32, i64, _dp, _dp, _dp, _dp, i32, i64, i64, i64, i32, _dp]
+        L.orc_type1_at.argtypes = [
+            i32, i64, _dp, _dp, _dp, _dp, i32, i64, i64, i64, i32, i64, _i64p, _dp,
+        ]
+        L.orc_type2.argtypes = [i32, i64, _dp, _dp, _dp, _dp, i32, i64, i64, i64, i32, _dp]
+        L.orc_type3.argtypes = [i32, i64, _dp, _dp, _dp, _dp, i32, i64, _dp, _dp, _dp, _dp]
+        L.fcpu_setup.argtypes = [dbl, dbl, ctypes.POINTER(i32), _dp]
+        L.fcpu_setup.restype = i32
+        L.fcpu_next235even.argtypes = [i64]
+        L.fcpu_next235even.restype = i64
+        L.fcpu_fine_size.argtypes = [i64, dbl, i32]
+        L.fcpu_fine_size.restype = i64
+        L.fcpu_gauss_legendre.argtypes = [i32, _dp, _dp]
+        L.fcpu_phihat.argtypes = [i32, dbl, i64, i64, i64, _dp]
+        L.fcpu_spread.argtypes = [i32, i64, _dp, _dp, _dp, _dp, i32, dbl, i64, i64, i64, _dp]
+        L.fcpu_interp.argtypes = [i32, i64, _dp, _dp, _dp, _dp, i32, dbl, i64, i64, i64, _dp]
+        L.fcpu_deconv.argtypes = [
+            i32, i64, i64, i64, i64, i64, i64, _dp, _dp, _dp, i32, _dp, _dp,
+        ]
+        L.fcpu_amplify.argtypes = [
+            i32, i64, i64, i64, i64, i64, i64, _dp, _dp, _dp, i32, _dp, _dp,
+        ]
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _native_points(points):
+    """User-order point list -> (dim, x, y, z) in native x-fastest order."""
+    pts = [_f64(p).ravel() for p in points][::-1]
+    dim = len(pts)
+    while len(pts) < 3:
+        pts.append(pts[0])
+    return dim, pts
+
+
+def _native_modes(n_modes):
+    n = list(n_modes)[::-1]
+    while len(n) < 3:
+        n.append(1)
+    return n
+
+
+def type1(points, c, n_modes, iflag, modeord=0):
+    """f[k] = sum_j c_j exp(i iflag k.x_j); returns a C-ordered array of shape n_modes."""
+    dim, (x, y, z) = _native_points(points)
+    c = _c128(c).ravel()
+    n1, n2, n3 = _native_modes(n_modes)
+    out = np.zeros(tuple(n_modes), dtype=np.complex128)
+    lib().orc_type1(
+        dim, c.size, _ptr(x), _ptr(y), _ptr(z), _ptr(c.view(np.float64)), int(iflag),
+        n1, n2, n3, int(modeord), _ptr(out.view(np.float64)),
+    )
+    return out
+
+
+def type1_at(points, c, n_modes, iflag, flat_index, modeord=0):
+    """Type 1 at selected C-order flat indices of the mode array."""
+    dim, (x, y, z) = _native_points(points)
+    c = _c128(c).ravel()
+    n1, n2, n3 = _native_modes(n_modes)
+    # a C-order flat index over n_modes is the native (x-fastest) flat index too
+    kidx = np.ascontiguousarray(flat_index, dtype=np.int64).ravel()
+    out = np.zeros(kidx.size, dtype=np.complex128)
+    lib().orc_type1_at(
+        dim, c.size, _ptr(x), _ptr(y), _ptr(z), _ptr(c.view(np.float64)), int(iflag),
+        n1, n2, n3, int(modeord), kidx.size, kidx.ctypes.data_as(_i64p),
+        _ptr(out.view(np.float64)),
+    )
+    return out
+
+
+def type2(f, points, iflag, modeord=0):
+    """c_j = sum_k f[k] exp(i iflag k.x_j) for a C-ordered mode array f."""
+    dim, (x, y, z) = _native_points(points)
+    f = _c128(f)
+    n1, n2, n3 = _native_modes(f.shape)
+    M = x.size
+    out = np.zeros(M, dtype=np.complex128)
+    lib().orc_type2(
+        dim, M, _ptr(x), _ptr(y), _ptr(z), _ptr(f.view(np.float64)), int(iflag),
+        n1, n2, n3, int(modeord), _ptr(out.view(np.float64)),
+    )
+    return out
+
+
+def type3(c, points, targets, iflag):
+    """f_k = sum_j c_j exp(i iflag s_k.x_j)."""
+    dim, (x, y, z) = _native_points(points)
+    dim2, (s, t, u) = _native_points(targets)
+    assert dim == dim2
+    c = _c128(c).ravel()
+    N = s.size
+    out = np.zeros(N, dtype=np.complex128)
+    lib().orc_type3(
+        dim, c.size, _ptr(x), _ptr(y), _ptr(z), _ptr(c.view(np.float64)), int(iflag),
+        N, _ptr(s), _ptr(t), _ptr(u), _ptr(out.view(np.float64)),
+    )
+    return out
+
+
+def rel_l2(a, b):
+    """Relative l2 error ||a-b|| / ||b||."""
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))

@@ -1,0 +1,133 @@
+/*
+ * FP64 direct-sum oracle for the three NUFFT types.
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the checker. Only tests/, the smoke
+ * check and the CPU-baseline leg of bench.py load it; the product library never
+ * links or calls it.
+ *
+ * Definitions follow the reference's own closed forms:
+ *   type 1  f_k = sum_j c_j exp(i*iflag*k.x_j)      tests/ops_test.py:42-45
+ *   type 2  c_j = sum_k f_k exp(i*iflag*k.x_j)      tests/ops_test.py:77-84
+ *   type 3  f_k = sum_j c_j exp(i*iflag*s_k.x_j)    tests/ops_test.py:112-116
+ * Mode index <-> frequency maps follow get_frequency_array
+ * (src/jax_finufft/ops.py:114-123): modeord 0 is k = i - n/2 (floor),
+ * modeord 1 is FFT order.
+ *
+ * Axis convention is the native one the FFI layer receives after the lowering
+ * reverses the user's axes (src/jax_finufft/lowering.py:266-275): x is the
+ * fastest-varying axis of the mode array, f[i1 + n1*(i2 + n2*i3)].
+ *
+ * Complex arrays are interleaved (re, im) doubles.  Phases are evaluated with
+ * libm sincos on each term (no recurrences) and summed with Neumaier
+ * compensation, so the result is accurate to ~1e-15 relative to sum |c_j|.
+ */
+#include <math.h>
+#include <stdint.h>
+
+static inline double mode_of(int64_t i, int64_t n, int modeord) {
+    if (modeord == 0) return (double)(i - n / 2);
+    return (double)((i < (n + 1) / 2) ? i : i - n);
+}
+
+typedef struct { double s, c; } kahan_t;
+
+static inline void kadd(kahan_t *a, double v) {
+    double t = a->s + v;
+    if (fabs(a->s) >= fabs(v)) a->c += (a->s - t) + v;
+    else a->c += (v - t) + a->s;
+    a->s = t;
+}
+
+/* Type 1, evaluated at an explicit list of flat output indices (nout of them).
+ * out[2*q], out[2*q+1] receive f at flat index kidx[q]. */
+void orc_type1_at(int dim, int64_t M, const double *x, const double *y, const double *z,
+                  const double *c, int iflag, int64_t n1, int64_t n2, int64_t n3,
+                  int modeord, int64_t nout, const int64_t *kidx, double *out) {
+    if (dim < 2) n2 = 1;
+    if (dim < 3) n3 = 1;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t q = 0; q < nout; q++) {
+        int64_t k = kidx[q];
+        int64_t i1 = k % n1, i2 = (k / n1) % n2, i3 = k / (n1 * n2);
+        double k1 = mode_of(i1, n1, modeord);
+        double k2 = dim > 1 ? mode_of(i2, n2, modeord) : 0.0;
+        double k3 = dim > 2 ? mode_of(i3, n3, modeord) : 0.0;
+        kahan_t re = {0, 0}, im = {0, 0};
+        for (int64_t j = 0; j < M; j++) {
+            double ph = k1 * x[j];
+            if (dim > 1) ph += k2 * y[j];
+            if (dim > 2) ph += k3 * z[j];
+            ph *= (double)iflag;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            double cr = c[2 * j], ci = c[2 * j + 1];
+            kadd(&re, cr * cs - ci * sn);
+            kadd(&im, cr * sn + ci * cs);
+        }
+        out[2 * q] = re.s + re.c;
+        out[2 * q + 1] = im.s + im.c;
+    }
+}
+
+/* Type 1 over the full mode array. */
+void orc_type1(int dim, int64_t M, const double *x, const double *y, const double *z,
+               const double *c, int iflag, int64_t n1, int64_t n2, int64_t n3, int modeord,
+               double *f) {
+    if (dim < 2) n2 = 1;
+    if (dim < 3) n3 = 1;
+    int64_t N = n1 * n2 * n3;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < N; k++) {
+        orc_type1_at(dim, M, x, y, z, c, iflag, n1, n2, n3, modeord, 1, &k, f + 2 * k);
+    }
+}
+
+/* Type 2: c_j for every point j. */
+void orc_type2(int dim, int64_t M, const double *x, const double *y, const double *z,
+               const double *f, int iflag, int64_t n1, int64_t n2, int64_t n3, int modeord,
+               double *c) {
+    if (dim < 2) n2 = 1;
+    if (dim < 3) n3 = 1;
+    int64_t N = n1 * n2 * n3;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t j = 0; j < M; j++) {
+        kahan_t re = {0, 0}, im = {0, 0};
+        for (int64_t k = 0; k < N; k++) {
+            int64_t i1 = k % n1, i2 = (k / n1) % n2, i3 = k / (n1 * n2);
+            double ph = mode_of(i1, n1, modeord) * x[j];
+            if (dim > 1) ph += mode_of(i2, n2, modeord) * y[j];
+            if (dim > 2) ph += mode_of(i3, n3, modeord) * z[j];
+            ph *= (double)iflag;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            double fr = f[2 * k], fi = f[2 * k + 1];
+            kadd(&re, fr * cs - fi * sn);
+            kadd(&im, fr * sn + fi * cs);
+        }
+        c[2 * j] = re.s + re.c;
+        c[2 * j + 1] = im.s + im.c;
+    }
+}
+
+/* Type 3: f_k = sum_j c_j exp(i*iflag*(s_k x_j + t_k y_j + u_k z_j)). */
+void orc_type3(int dim, int64_t M, const double *x, const double *y, const double *z,
+               const double *c, int iflag, int64_t N, const double *s, const double *t,
+               const double *u, double *f) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t k = 0; k < N; k++) {
+        kahan_t re = {0, 0}, im = {0, 0};
+        for (int64_t j = 0; j < M; j++) {
+            double ph = s[k] * x[j];
+            if (dim > 1) ph += t[k] * y[j];
+            if (dim > 2) ph += u[k] * z[j];
+            ph *= (double)iflag;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            double cr = c[2 * j], ci = c[2 * j + 1];
+            kadd(&re, cr * cs - ci * sn);
+            kadd(&im, cr * sn + ci * cs);
+        }
+        f[2 * k] = re.s + re.c;
+        f[2 * k + 1] = im.s + im.c;
+    }
+}
