@@ -1,0 +1,198 @@
+/*
+ * jax_finufft_amd: MI355X (gfx950) NUFFT backend, C ABI.
+ *
+ * Two layers, both plain C with pointers and sizes:
+ *
+ * 1. The plan interface (jfa_makeplan / jfa_setpts / jfa_execute /
+ *    jfa_destroy).  It replaces the cuFINUFFT guru calls that the reference's
+ *    host driver makes through its wrapper:
+ *      makeplan  lib/cufinufft_wrapper.cc:48-61   (called at lib/kernels.cc.cu:51)
+ *      setpts    lib/cufinufft_wrapper.cc:63-73   (called at lib/kernels.cc.cu:63-68)
+ *      execute   lib/cufinufft_wrapper.cc:75-89   (called at lib/kernels.cc.cu:75)
+ *      destroy   lib/cufinufft_wrapper.cc:91-99   (called at lib/kernels.cc.cu:84)
+ *      default_opts / update_opts  lib/cufinufft_wrapper.cc:11-46
+ *
+ * 2. Eighteen transform entry points, one per XLA FFI handler of the
+ *    reference (lib/jax_finufft_gpu.cc:363-384), with the same attribute list
+ *    in the same order (lib/jax_finufft_gpu.cc:28-60) and the same operand
+ *    order (source, points..., [targets...], output; points and n_k already
+ *    reversed into x-fastest order by the lowering,
+ *    src/jax_finufft/lowering.py:266-292).  Each replaces one
+ *    nufft{d}d{t}{f}_impl of lib/kernels.cc.cu:117-215 + run_nufft
+ *    (lib/kernels.cc.cu:25-92).  Unlike the reference they take plans from a
+ *    per-device cache and never block the host: all work is queued on
+ *    `stream`.
+ *
+ * All device pointers are HIP device memory.  Complex arrays are interleaved
+ * (re, im) in the transform's precision.  Single precision entry points end in
+ * "f" and take float points and complex64 data; the others take double.
+ *
+ * Return codes: 0 success, 1 the eps-too-small warning (not an error, as in
+ * lib/kernels.cc.cu:52-55), >= 2 an error; jfa_last_error() describes the
+ * last error raised on the calling thread.
+ */
+#ifndef JAX_FINUFFT_AMD_H_
+#define JAX_FINUFFT_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A hipStream_t, passed as an opaque pointer (NULL = the null stream). */
+typedef void *jfa_stream_t;
+
+typedef struct jfa_plan_s *jfa_plan;
+
+enum {
+    JFA_OK = 0,
+    JFA_WARN_EPS_TOO_SMALL = 1,
+    JFA_ERR_BAD_ARG = 2,
+    JFA_ERR_ALLOC = 3,
+    JFA_ERR_HIP = 4,
+    JFA_ERR_FFT = 5,
+    JFA_ERR_NOT_SUPPORTED = 6,
+    JFA_ERR_TOO_LARGE = 7
+};
+
+/* Spread/interp method (gpu_method): 0 auto (autotuned), 1 global-memory
+ * atomics in sorted order, 2 LDS subproblem tiles. */
+enum { JFA_METHOD_AUTO = 0, JFA_METHOD_GM_SORT = 1, JFA_METHOD_SM = 2 };
+
+/* Options.  The first seven fields are the ones the reference forwards to the
+ * GPU (src/jax_finufft/options.py:105-119, lowering.py:337-343); the rest are
+ * the GPU fields it declares (options.py:66-79) plus stream/device and two
+ * additions of this backend. */
+typedef struct {
+    int modeord;           /* 0: CMCL order -N/2..(N-1)/2; 1: FFT order */
+    double upsampfac;      /* sigma; 2.0 default (gpu_upsampfac) */
+    int gpu_method;        /* JFA_METHOD_* */
+    int gpu_sort;          /* 1: sort points into bins (always used by SM) */
+    int gpu_kerevalmeth;   /* 1: Horner piecewise polynomial (0: exp/sqrt) */
+    int gpu_maxbatchsize;  /* transforms per spread/FFT batch; 0 = all */
+    int debug;             /* 1: print per-stage timings to stderr */
+    int gpu_binsizex, gpu_binsizey, gpu_binsizez; /* 0 = default/autotuned */
+    int gpu_maxsubprobsize;  /* points per LDS subproblem; 0 = default */
+    int gpu_spreadinterponly; /* 1: execute spreads/interpolates only; the mode
+                                 array is then the fine grid (nf = nmodes) */
+    int gpu_device_id;     /* -1: current device */
+    jfa_stream_t gpu_stream;
+    int width_rule;        /* 0: width reaches eps (w = ceil(log10(20/eps)));
+                              1: FINUFFT's published rule w = ceil(log10(10/eps)) */
+    int kernel_width;      /* >0: force this width (2..16) */
+} jfa_opts;
+
+void jfa_default_opts(jfa_opts *opts);
+
+/* type 1, 2 or 3; dim 1..3; nmodes[3] in x-fastest order (ignored for type 3);
+ * iflag sign of the exponent; ntransf stacked transforms sharing the points. */
+int jfa_makeplan(int type, int dim, const int64_t *nmodes, int iflag, int ntransf, double eps,
+                 int is_double, const jfa_opts *opts, jfa_plan *plan_out);
+
+/* M points (x, y, z device arrays of length M; unused ones may be NULL).
+ * Type 3 also takes N target frequencies s, t, u.  Points may be any real
+ * value (they are folded periodically). */
+int jfa_setpts(jfa_plan plan, int64_t M, const void *x, const void *y, const void *z, int64_t N,
+               const void *s, const void *t, const void *u);
+
+/* Type 1: c (ntransf x M) -> fk (ntransf x modes); type 2: fk -> c;
+ * type 3: c (ntransf x M) -> fk (ntransf x N). */
+int jfa_execute(jfa_plan plan, void *c, void *fk);
+
+/* Queue subsequent work of this plan on `stream`, ordered after the plan's
+ * previous work on whatever stream it used before. */
+int jfa_set_stream(jfa_plan plan, jfa_stream_t stream);
+
+int jfa_destroy(jfa_plan plan);
+
+/* Kernel parameters the plan chose: width, beta, fine grid (x-fastest). */
+int jfa_plan_info(jfa_plan plan, int *ns, double *beta, int64_t *nf /* [3] */);
+
+const char *jfa_last_error(void);
+
+/* Number of plans held by the per-device plan cache, and a way to drop them. */
+int jfa_cache_size(void);
+void jfa_cache_clear(void);
+
+/* Host-only plan-time mathematics (no device work; usable without a GPU).
+ * They expose what makeplan computes inside cuFINUFFT in the reference
+ * (called at lib/kernels.cc.cu:51), so it can be checked on its own.
+ *   jfa_kernel_params: width ns, shape beta and Horner coefficient count nc for
+ *     eps at upsampling factor upsampfac; returns 1 if eps was clamped.
+ *   jfa_fine_grid_size: fine grid for n modes (next even 2,3,5-smooth >=
+ *     max(upsampfac n, 2 ns)).
+ *   jfa_horner_table: nc x ns monomial coefficients, power-major.
+ *   jfa_kernel_fourier: psi_hat(2 pi k / nf) for k = kmin .. kmin+n-1. */
+int jfa_kernel_params(double eps, double upsampfac, int width_rule, int is_double, int *ns,
+                      double *beta, int *nc);
+int64_t jfa_fine_grid_size(int64_t n, double upsampfac, int ns);
+int jfa_horner_table(int ns, int nc, double beta, double *coef);
+int jfa_kernel_fourier(int ns, double beta, int64_t nf, int64_t kmin, int64_t n, double *out);
+
+/* ------------------------------------------------------------------------
+ * Transform entry points, one per XLA FFI handler of the reference
+ * (lib/jax_finufft_gpu.cc:363-384).  Attribute order matches
+ * NUFFT_COMMON_ATTRS_{FLOAT,DOUBLE} (lib/jax_finufft_gpu.cc:28-60).
+ * n_tot independent problems, each with n_transf stacked transforms over n_j
+ * points; problem `idx` reads points at idx*n_j, strengths at
+ * idx*n_j*n_transf and writes modes at idx*n_k_total*n_transf
+ * (lib/kernels.cc.cu:57-60).  Type 3 reads n_k_1 targets per problem at
+ * idx*n_k_1 (lib/kernels.cc.cu:66).
+ * ------------------------------------------------------------------------ */
+#define JFA_ATTRS_F                                                                      \
+    jfa_stream_t stream, float eps, int64_t iflag, int64_t n_tot, int64_t n_transf,      \
+        int64_t n_j, int64_t n_k_1, int64_t n_k_2, int64_t n_k_3, int64_t modeord,       \
+        double upsampfac, int64_t gpu_method, int64_t gpu_sort, int64_t gpu_kerevalmeth, \
+        int64_t gpu_maxbatchsize, int64_t debug
+#define JFA_ATTRS_D                                                                      \
+    jfa_stream_t stream, double eps, int64_t iflag, int64_t n_tot, int64_t n_transf,     \
+        int64_t n_j, int64_t n_k_1, int64_t n_k_2, int64_t n_k_3, int64_t modeord,       \
+        double upsampfac, int64_t gpu_method, int64_t gpu_sort, int64_t gpu_kerevalmeth, \
+        int64_t gpu_maxbatchsize, int64_t debug
+
+/* type 1: source (n_tot, n_transf, n_j) -> output (n_tot, n_transf, modes) */
+int jfa_nufft1d1f(JFA_ATTRS_F, const void *source, const void *points_x, void *output);
+int jfa_nufft1d1(JFA_ATTRS_D, const void *source, const void *points_x, void *output);
+int jfa_nufft2d1f(JFA_ATTRS_F, const void *source, const void *points_x, const void *points_y,
+                  void *output);
+int jfa_nufft2d1(JFA_ATTRS_D, const void *source, const void *points_x, const void *points_y,
+                 void *output);
+int jfa_nufft3d1f(JFA_ATTRS_F, const void *source, const void *points_x, const void *points_y,
+                  const void *points_z, void *output);
+int jfa_nufft3d1(JFA_ATTRS_D, const void *source, const void *points_x, const void *points_y,
+                 const void *points_z, void *output);
+
+/* type 2: source (n_tot, n_transf, modes) -> output (n_tot, n_transf, n_j) */
+int jfa_nufft1d2f(JFA_ATTRS_F, const void *source, const void *points_x, void *output);
+int jfa_nufft1d2(JFA_ATTRS_D, const void *source, const void *points_x, void *output);
+int jfa_nufft2d2f(JFA_ATTRS_F, const void *source, const void *points_x, const void *points_y,
+                  void *output);
+int jfa_nufft2d2(JFA_ATTRS_D, const void *source, const void *points_x, const void *points_y,
+                 void *output);
+int jfa_nufft3d2f(JFA_ATTRS_F, const void *source, const void *points_x, const void *points_y,
+                  const void *points_z, void *output);
+int jfa_nufft3d2(JFA_ATTRS_D, const void *source, const void *points_x, const void *points_y,
+                 const void *points_z, void *output);
+
+/* type 3: source (n_tot, n_transf, n_j) -> output (n_tot, n_transf, n_k_1) */
+int jfa_nufft1d3f(JFA_ATTRS_F, const void *source, const void *points_x, const void *target_x,
+                  void *output);
+int jfa_nufft1d3(JFA_ATTRS_D, const void *source, const void *points_x, const void *target_x,
+                 void *output);
+int jfa_nufft2d3f(JFA_ATTRS_F, const void *source, const void *points_x, const void *points_y,
+                  const void *target_x, const void *target_y, void *output);
+int jfa_nufft2d3(JFA_ATTRS_D, const void *source, const void *points_x, const void *points_y,
+                 const void *target_x, const void *target_y, void *output);
+int jfa_nufft3d3f(JFA_ATTRS_F, const void *source, const void *points_x, const void *points_y,
+                  const void *points_z, const void *target_x, const void *target_y,
+                  const void *target_z, void *output);
+int jfa_nufft3d3(JFA_ATTRS_D, const void *source, const void *points_x, const void *points_y,
+                 const void *points_z, const void *target_x, const void *target_y,
+                 const void *target_z, void *output);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAX_FINUFFT_AMD_H_ */

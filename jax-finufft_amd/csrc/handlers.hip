@@ -1,0 +1,262 @@
+// The eighteen transform entry points (one per XLA FFI handler of the
+// reference, lib/jax_finufft_gpu.cc:363-384) and the per-device plan cache
+// behind them.
+//
+// Each entry point does what nufft{1,2,3}_impl + run_nufft do in the reference
+// (lib/kernels.cc.cu:25-215): build options from the attributes
+// (build_opts, :98-111), get a plan, then for every index < n_tot set the
+// points and execute with the reference's offsets (:57-80).  The differences:
+// the plan comes from a cache keyed by everything makeplan depends on instead
+// of being created and destroyed per call (:51, :84), and nothing waits for
+// the device (the reference synchronises at :83).
+//
+// Thread safety: the cache is mutex-protected; a plan is checked out by one
+// call at a time, and a call on a different stream than the plan's last one
+// is ordered after it with an event (jfa_set_stream).
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "plan_state.hpp"
+
+using namespace jfa;
+
+namespace {
+
+struct Key {
+    int device, type, dim, is_double, iflag, ntransf;
+    int64_t n[3];
+    double eps, upsampfac;
+    int modeord, gpu_method, gpu_sort, kerevalmeth, maxbatch, debug;
+
+    bool operator==(const Key &o) const {
+        return device == o.device && type == o.type && dim == o.dim &&
+               is_double == o.is_double && iflag == o.iflag && ntransf == o.ntransf &&
+               n[0] == o.n[0] && n[1] == o.n[1] && n[2] == o.n[2] && eps == o.eps &&
+               upsampfac == o.upsampfac && modeord == o.modeord && gpu_method == o.gpu_method &&
+               gpu_sort == o.gpu_sort && kerevalmeth == o.kerevalmeth &&
+               maxbatch == o.maxbatch && debug == o.debug;
+    }
+};
+
+struct Entry {
+    Key key;
+    jfa_plan plan;
+    bool busy;
+    uint64_t stamp;
+};
+
+std::mutex g_mu;
+std::vector<Entry> g_cache;
+uint64_t g_clock = 0;
+constexpr size_t kMaxIdlePlans = 16;
+
+jfa_plan checkout(const Key &k, const jfa_opts &o, int *rc) {
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (auto &e : g_cache) {
+            if (!e.busy && e.key == k) {
+                e.busy = true;
+                e.stamp = ++g_clock;
+                *rc = JFA_OK;
+                return e.plan;
+            }
+        }
+    }
+    jfa_plan p = nullptr;
+    *rc = jfa_makeplan(k.type, k.dim, k.n, k.iflag, k.ntransf, k.eps, k.is_double, &o, &p);
+    if (*rc > JFA_WARN_EPS_TOO_SMALL) return nullptr;  // eps-too-small is only a warning
+    *rc = JFA_OK;
+    std::vector<jfa_plan> evict;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_cache.push_back({k, p, true, ++g_clock});
+        size_t idle = 0;
+        for (auto &e : g_cache) idle += !e.busy;
+        while (idle > kMaxIdlePlans) {
+            auto lru = g_cache.end();
+            for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+                if (!it->busy && (lru == g_cache.end() || it->stamp < lru->stamp)) lru = it;
+            evict.push_back(lru->plan);
+            g_cache.erase(lru);
+            --idle;
+        }
+    }
+    for (jfa_plan q : evict) jfa_destroy(q);
+    return p;
+}
+
+void checkin(jfa_plan p) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto &e : g_cache)
+        if (e.plan == p) e.busy = false;
+}
+
+template <typename T>
+int run(int type, int dim, jfa_stream_t stream, double eps, int64_t iflag, int64_t n_tot,
+        int64_t n_transf, int64_t n_j, int64_t n_k_1, int64_t n_k_2, int64_t n_k_3,
+        int64_t modeord, double upsampfac, int64_t gpu_method, int64_t gpu_sort,
+        int64_t gpu_kerevalmeth, int64_t gpu_maxbatchsize, int64_t debug, const void *source,
+        const void *const pts[3], const void *const tgt[3], void *output) {
+    if (n_tot < 0 || n_transf < 1 || n_j < 0 || n_transf > INT32_MAX) {
+        set_error("bad sizes: n_tot=%lld n_transf=%lld n_j=%lld", (long long)n_tot,
+                  (long long)n_transf, (long long)n_j);
+        return JFA_ERR_BAD_ARG;
+    }
+    const int64_t nk[3] = {n_k_1, n_k_2, n_k_3};
+    Key k;
+    std::memset(&k, 0, sizeof(k));
+    if (hipGetDevice(&k.device) != hipSuccess) k.device = 0;
+    k.type = type;
+    k.dim = dim;
+    k.is_double = sizeof(T) == 8;
+    k.iflag = iflag >= 0 ? 1 : -1;
+    k.ntransf = (int)n_transf;
+    for (int d = 0; d < 3; ++d) k.n[d] = (type != 3 && d < dim) ? nk[d] : 1;
+    k.eps = eps;
+    k.upsampfac = upsampfac;
+    k.modeord = (int)modeord;
+    k.gpu_method = (int)gpu_method;
+    k.gpu_sort = (int)gpu_sort;
+    k.kerevalmeth = (int)gpu_kerevalmeth;
+    k.maxbatch = (int)gpu_maxbatchsize;
+    k.debug = (int)debug;
+    int64_t n_k_total = 1;
+    if (type == 3) {
+        n_k_total = n_k_1;
+    } else {
+        for (int d = 0; d < dim; ++d) {
+            if (nk[d] < 1) {
+                set_error("n_k_%d must be >= 1", d + 1);
+                return JFA_ERR_BAD_ARG;
+            }
+            n_k_total *= nk[d];
+        }
+    }
+    if (n_tot == 0) return JFA_OK;
+
+    jfa_opts o;
+    jfa_default_opts(&o);
+    o.modeord = k.modeord;
+    o.upsampfac = upsampfac;
+    o.gpu_method = k.gpu_method;
+    o.gpu_sort = k.gpu_sort;
+    o.gpu_kerevalmeth = k.kerevalmeth;
+    o.gpu_maxbatchsize = k.maxbatch;
+    o.debug = k.debug;
+    o.gpu_stream = stream;
+
+    int rc = JFA_OK;
+    jfa_plan p = checkout(k, o, &rc);
+    if (!p) return rc;
+    rc = jfa_set_stream(p, stream);
+    const size_t cplx = 2 * sizeof(T);
+    for (int64_t idx = 0; idx < n_tot && rc == JFA_OK; ++idx) {
+        const int64_t i = idx * n_j;
+        const int64_t j = idx * n_j * n_transf;
+        const int64_t kk = idx * n_k_total * n_transf;
+        const T *x = (const T *)pts[0] + i;
+        const T *y = dim > 1 ? (const T *)pts[1] + i : nullptr;
+        const T *z = dim > 2 ? (const T *)pts[2] + i : nullptr;
+        if (type == 3) {
+            const int64_t it = idx * n_k_total;
+            const T *s = (const T *)tgt[0] + it;
+            const T *t = dim > 1 ? (const T *)tgt[1] + it : nullptr;
+            const T *u = dim > 2 ? (const T *)tgt[2] + it : nullptr;
+            rc = jfa_setpts(p, n_j, x, y, z, n_k_total, s, t, u);
+        } else {
+            rc = jfa_setpts(p, n_j, x, y, z, 0, nullptr, nullptr, nullptr);
+        }
+        if (rc) break;
+        char *src = (char *)source;
+        char *out = (char *)output;
+        if (type == 2)  // c is the output, the modes are the source
+            rc = jfa_execute(p, out + cplx * j, src + cplx * kk);
+        else
+            rc = jfa_execute(p, src + cplx * j, out + cplx * kk);
+    }
+    checkin(p);
+    if (rc) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("HIP error: %s", hipGetErrorString(e));
+        return JFA_ERR_HIP;
+    }
+    return JFA_OK;
+}
+
+}  // namespace
+
+extern "C" int jfa_cache_size(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return (int)g_cache.size();
+}
+
+extern "C" void jfa_cache_clear(void) {
+    std::vector<jfa_plan> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (auto it = g_cache.begin(); it != g_cache.end();) {
+            if (!it->busy) {
+                drop.push_back(it->plan);
+                it = g_cache.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    for (jfa_plan p : drop) jfa_destroy(p);
+}
+
+#define JFA_FWD_ATTRS                                                                     \
+    stream, (double)eps, iflag, n_tot, n_transf, n_j, n_k_1, n_k_2, n_k_3, modeord,       \
+        upsampfac, gpu_method, gpu_sort, gpu_kerevalmeth, gpu_maxbatchsize, debug
+
+#define JFA_DEFINE_12(NAME, ATTRS, T, TYPE, DIM, ...)                                    \
+    extern "C" int NAME(ATTRS, const void *source, __VA_ARGS__, void *output) {           \
+        const void *const pts[3] = {JFA_PTS_##DIM};                                       \
+        const void *const tgt[3] = {nullptr, nullptr, nullptr};                           \
+        return run<T>(TYPE, DIM, JFA_FWD_ATTRS, source, pts, tgt, output);               \
+    }
+#define JFA_PTS_1 points_x, nullptr, nullptr
+#define JFA_PTS_2 points_x, points_y, nullptr
+#define JFA_PTS_3 points_x, points_y, points_z
+
+#define JFA_DEFINE_3(NAME, ATTRS, T, DIM, ...)                                           \
+    extern "C" int NAME(ATTRS, const void *source, __VA_ARGS__, void *output) {           \
+        const void *const pts[3] = {JFA_PTS_##DIM};                                       \
+        const void *const tgt[3] = {JFA_TGT_##DIM};                                       \
+        return run<T>(3, DIM, JFA_FWD_ATTRS, source, pts, tgt, output);                  \
+    }
+#define JFA_TGT_1 target_x, nullptr, nullptr
+#define JFA_TGT_2 target_x, target_y, nullptr
+#define JFA_TGT_3 target_x, target_y, target_z
+
+#define P1 const void *points_x
+#define P2 const void *points_x, const void *points_y
+#define P3 const void *points_x, const void *points_y, const void *points_z
+#define T1 const void *target_x
+#define T2 const void *target_x, const void *target_y
+#define T3 const void *target_x, const void *target_y, const void *target_z
+
+JFA_DEFINE_12(jfa_nufft1d1f, JFA_ATTRS_F, float, 1, 1, P1)
+JFA_DEFINE_12(jfa_nufft1d1, JFA_ATTRS_D, double, 1, 1, P1)
+JFA_DEFINE_12(jfa_nufft2d1f, JFA_ATTRS_F, float, 1, 2, P2)
+JFA_DEFINE_12(jfa_nufft2d1, JFA_ATTRS_D, double, 1, 2, P2)
+JFA_DEFINE_12(jfa_nufft3d1f, JFA_ATTRS_F, float, 1, 3, P3)
+JFA_DEFINE_12(jfa_nufft3d1, JFA_ATTRS_D, double, 1, 3, P3)
+
+JFA_DEFINE_12(jfa_nufft1d2f, JFA_ATTRS_F, float, 2, 1, P1)
+JFA_DEFINE_12(jfa_nufft1d2, JFA_ATTRS_D, double, 2, 1, P1)
+JFA_DEFINE_12(jfa_nufft2d2f, JFA_ATTRS_F, float, 2, 2, P2)
+JFA_DEFINE_12(jfa_nufft2d2, JFA_ATTRS_D, double, 2, 2, P2)
+JFA_DEFINE_12(jfa_nufft3d2f, JFA_ATTRS_F, float, 2, 3, P3)
+JFA_DEFINE_12(jfa_nufft3d2, JFA_ATTRS_D, double, 2, 3, P3)
+
+JFA_DEFINE_3(jfa_nufft1d3f, JFA_ATTRS_F, float, 1, P1, T1)
+JFA_DEFINE_3(jfa_nufft1d3, JFA_ATTRS_D, double, 1, P1, T1)
+JFA_DEFINE_3(jfa_nufft2d3f, JFA_ATTRS_F, float, 2, P2, T2)
+JFA_DEFINE_3(jfa_nufft2d3, JFA_ATTRS_D, double, 2, P2, T2)
+JFA_DEFINE_3(jfa_nufft3d3f, JFA_ATTRS_F, float, 3, P3, T3)
+JFA_DEFINE_3(jfa_nufft3d3, JFA_ATTRS_D, double, 3, P3, T3)

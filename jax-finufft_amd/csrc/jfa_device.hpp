@@ -1,0 +1,118 @@
+// Device helpers shared by the setpts / spread / interp kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "jfa_internal.hpp"
+
+namespace jfa {
+
+// Fold a coordinate to the periodic fine grid and locate its tap window.
+// u = (x - shift) * scale mod nf in [0, nf); the ns taps are l0 .. l0+ns-1 with
+// l0 = ceil(u - ns/2); frac = l0 - (u - ns/2) in [0, 1).  Everything is done in
+// double so that the integer first tap is the same wherever it is computed
+// (bin counting and the sorted scatter both call this one function).
+struct Folded {
+    int l0w;     // first tap wrapped into [0, nf)
+    double t;    // Horner variable 2*frac - 1 in [-1, 1)
+};
+
+__device__ __forceinline__ Folded fold_point(double x, int64_t nf, double scale, double shift,
+                                             int ns) {
+    const double dnf = (double)nf;
+    double u = (x - shift) * scale;
+    u -= dnf * floor(u / dnf);
+    if (u >= dnf) u -= dnf;
+    if (u < 0.0) u += dnf;
+    const double v = u - 0.5 * ns;
+    const double l0 = ceil(v);
+    Folded f;
+    int li = (int)l0;
+    f.l0w = li < 0 ? li + (int)nf : li;
+    f.t = 2.0 * (l0 - v) - 1.0;
+    return f;
+}
+
+// ns kernel values for one point along one dimension.  Tap i sits at offset
+// r_i = -ns/2 + i + (t+1)/2 cells from the point; its weight is
+// phi(2 r_i / ns).  HORNER: evaluate the plan-time fitted polynomials
+// (coef[p*NS + i] multiplies t^p); otherwise exp/sqrt directly.
+template <typename T, int NS, bool HORNER>
+__device__ __forceinline__ void eval_kernel(const T *__restrict__ coef, T beta, T t, T *ker) {
+    if constexpr (HORNER) {
+        constexpr int NC = horner_nc(NS);
+#pragma unroll
+        for (int i = 0; i < NS; ++i) ker[i] = coef[(NC - 1) * NS + i];
+#pragma unroll
+        for (int p = NC - 2; p >= 0; --p) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i) ker[i] = fma(ker[i], t, coef[p * NS + i]);
+        }
+    } else {
+        const T inv = T(2) / T(NS);
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            T z = (T(-0.5 * NS) + T(i) + T(0.5) * (t + T(1))) * inv;
+            T a = T(1) - z * z;
+            ker[i] = a < T(0) ? T(0) : exp(beta * (sqrt(a) - T(1)));
+        }
+    }
+}
+
+// Kernel values and their derivatives d/dr (per cell of offset) for one point
+// along one dimension: the derivative of the same fitted polynomial, times
+// dt/dr = 2.
+template <typename T, int NS>
+__device__ __forceinline__ void eval_kernel_deriv(const T *__restrict__ coef, T t, T *ker,
+                                                  T *dker) {
+    constexpr int NC = horner_nc(NS);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        ker[i] = coef[(NC - 1) * NS + i];
+        dker[i] = T(0);
+    }
+#pragma unroll
+    for (int p = NC - 2; p >= 0; --p) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            dker[i] = fma(dker[i], t, ker[i]);
+            ker[i] = fma(ker[i], t, coef[p * NS + i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) dker[i] *= T(2);
+}
+
+__device__ __forceinline__ void atomic_add_global(float *p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add_global(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+}  // namespace jfa
+
+// Expand BODY for the runtime width `ns` as a compile-time constant NS_.
+#define JFA_DISPATCH_NS(ns, ...)                                   \
+    switch (ns) {                                                  \
+        case 2: { constexpr int NS_ = 2; __VA_ARGS__; } break;     \
+        case 3: { constexpr int NS_ = 3; __VA_ARGS__; } break;     \
+        case 4: { constexpr int NS_ = 4; __VA_ARGS__; } break;     \
+        case 5: { constexpr int NS_ = 5; __VA_ARGS__; } break;     \
+        case 6: { constexpr int NS_ = 6; __VA_ARGS__; } break;     \
+        case 7: { constexpr int NS_ = 7; __VA_ARGS__; } break;     \
+        case 8: { constexpr int NS_ = 8; __VA_ARGS__; } break;     \
+        case 9: { constexpr int NS_ = 9; __VA_ARGS__; } break;     \
+        case 10: { constexpr int NS_ = 10; __VA_ARGS__; } break;   \
+        case 11: { constexpr int NS_ = 11; __VA_ARGS__; } break;   \
+        case 12: { constexpr int NS_ = 12; __VA_ARGS__; } break;   \
+        case 13: { constexpr int NS_ = 13; __VA_ARGS__; } break;   \
+        case 14: { constexpr int NS_ = 14; __VA_ARGS__; } break;   \
+        case 15: { constexpr int NS_ = 15; __VA_ARGS__; } break;   \
+        case 16: { constexpr int NS_ = 16; __VA_ARGS__; } break;   \
+        default: ::jfa::set_error("unsupported kernel width %d", ns); return JFA_ERR_BAD_ARG; \
+    }
+
+#define JFA_DISPATCH_DIM(dim, ...)                                 \
+    switch (dim) {                                                 \
+        case 1: { constexpr int DIM_ = 1; __VA_ARGS__; } break;    \
+        case 2: { constexpr int DIM_ = 2; __VA_ARGS__; } break;    \
+        case 3: { constexpr int DIM_ = 3; __VA_ARGS__; } break;    \
+        default: ::jfa::set_error("unsupported dimension %d", dim); return JFA_ERR_BAD_ARG; \
+    }

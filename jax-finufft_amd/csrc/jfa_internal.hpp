@@ -1,0 +1,142 @@
+// Internal declarations of the MI355X NUFFT backend: kernel-parameter math
+// (host), plan state, and the kernel launchers the plan drives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "jax_finufft_amd.h"
+
+namespace jfa {
+
+constexpr int kMaxNs = 16;  // widest ES kernel supported (taps per dimension)
+constexpr int kMaxNc = 19;  // most Horner coefficients per kernel interval
+
+// Horner coefficients per interval for width ns: degree ns+4, capped at 18.
+// (Chebyshev-interpolation error at this degree is ~exp(-beta)/2, i.e. below
+// the kernel's own truncation value.)
+constexpr int horner_nc(int ns) { return ns < 14 ? ns + 5 : kMaxNc; }
+
+// ------------------------------------------------------------ host math
+
+struct KernelParams {
+    int ns = 0;        // width in fine-grid cells
+    int nc = 0;        // Horner coefficients per interval
+    double beta = 0;   // ES shape parameter
+    int warn = 0;      // 1 = eps below what the width cap / precision reaches
+};
+
+// Width, beta for tolerance eps at upsampling factor sigma (params.cpp).
+KernelParams choose_kernel(double eps, double sigma, int rule, int is_double, int forced_ns);
+int64_t next235even(int64_t n);
+int64_t fine_size(int64_t n, double sigma, int ns);
+// Monomial Horner coefficients, power-major: coef[p*ns + i] multiplies t^p on
+// interval i (tap i of a point; t = 2*frac - 1 in [-1, 1)).
+void horner_fit(int ns, int nc, double beta, double *coef);
+// psi_hat(k h) for k = kmin .. kmin+n-1 on a fine grid of nf cells.
+void kernel_fourier(int ns, double beta, double nf, double kmin, int64_t n, double *out);
+// psi_hat at arbitrary frequencies omega (radians per fine-grid cell).
+void kernel_fourier_at(int ns, double beta, const double *omega, int64_t n, double *out);
+void gauss_legendre(int n, double *x, double *w);
+
+// ------------------------------------------------------------ errors
+
+void set_error(const char *fmt, ...);
+const char *last_error();
+
+#define JFA_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            ::jfa::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),      \
+                             __FILE__, __LINE__);                                        \
+            return JFA_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+// ------------------------------------------------------------ launch args
+
+// Geometry of the fine grid and of the point bins, passed by value to kernels.
+struct GridGeom {
+    int64_t nf[3];       // fine grid, x fastest (1 for unused dims)
+    double scale[3];     // grid units per coordinate unit: u = (x - shift) * scale
+    double shift[3];     // 0 for types 1/2; the type-3 centre otherwise
+    int bs[3];           // bin size in cells
+    int nb[3];           // bins per dimension
+    int64_t nbins;       // product of nb
+    int ns;              // kernel width
+    int dim;
+};
+
+// Points after setpts, in bin-sorted order: original index, first tap
+// (wrapped into [0, nf)) and the Horner variable t = 2*frac - 1 per dim.
+template <typename T>
+struct SortedPts {
+    int64_t M;
+    const int *idx;
+    const int *l0[3];
+    const T *t[3];
+};
+
+// Per-bin bookkeeping for the LDS-subproblem method.
+struct BinIndex {
+    const int *bin_start;  // nbins+1 exclusive prefix of counts
+    const int *sub_start;  // nbins+1 exclusive prefix of subproblems per bin
+    const int *nsub;       // total subproblems (device scalar)
+    int maxsub;
+};
+
+// Launchers (kernels_*.hip).  All queue work on `stream` and return a JFA code.
+template <typename T>
+int launch_fold_bin(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                    int *bin_of, int *rank_of, int *bin_count, hipStream_t stream);
+int launch_bin_scan(const GridGeom &g, const int *bin_count, int *bin_start, int *sub_start,
+                    int *nsub, int maxsub, hipStream_t stream);
+template <typename T>
+int launch_scatter(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                   const int *bin_of, const int *rank_of, const int *bin_start, int sorted,
+                   int *idx, int *l0, T *tv, hipStream_t stream);
+int launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
+
+template <typename T>
+int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
+                  const T *coef, int kerevalmeth, double beta, const T *c, int64_t c_stride,
+                  T *fw, int64_t fw_stride, int ntr, hipStream_t stream);
+template <typename T>
+int launch_interp(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
+                  const T *coef, int kerevalmeth, double beta, const T *fw, int64_t fw_stride,
+                  T *c, int64_t c_stride, int ntr, hipStream_t stream);
+template <typename T>
+int launch_deconv(int dim, const int64_t *n, const int64_t *nf, const T *dk0, const T *dk1,
+                  const T *dk2, int modeord, const T *fw, int64_t fw_stride, T *out,
+                  int64_t out_stride, int ntr, hipStream_t stream);
+template <typename T>
+int launch_amplify(int dim, const int64_t *n, const int64_t *nf, const T *dk0, const T *dk1,
+                   const T *dk2, int modeord, const T *f, int64_t f_stride, T *fw,
+                   int64_t fw_stride, int ntr, hipStream_t stream);
+
+// Type-3 helpers (kernels_type3.hip).
+// Per-array [min, max] of n values into out2 (device doubles).
+template <typename T>
+int launch_minmax(int64_t n, const T *a, double *out2, hipStream_t s);
+// cp[t][j] = c[t][j] * exp(i iflag sum_d D_d (x_dj - C_d))
+template <typename T>
+int launch_t3_prephase(int dim, int64_t M, int ntr, const T *x, const T *y, const T *z,
+                       const double *C, const double *D, int iflag, const T *c, T *cp,
+                       hipStream_t stream);
+// dec[k] = 1 / prod_d psi_hat(h_d gam_d (s_dk - D_d)) from a quadrature table
+// (wt[q], sn[q]): psi_hat(w) = ns sum_q wt[q] cos(w ns sn[q] / 2).
+template <typename T>
+int launch_t3_deconv_factors(int dim, int64_t N, const T *s, const T *t, const T *u,
+                             const double *D, const double *hg, int ns, int nq,
+                             const double *quad, T *dec, hipStream_t stream);
+// f[t][k] *= exp(i iflag sum_d s_dk C_d) * dec[k]
+template <typename T>
+int launch_t3_postphase(int dim, int64_t N, int ntr, const T *s, const T *t, const T *u,
+                        const double *C, int iflag, const T *dec, T *f, hipStream_t stream);
+
+}  // namespace jfa

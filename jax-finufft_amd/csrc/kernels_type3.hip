@@ -1,0 +1,205 @@
+// Type-3 kernels: extents of the point sets, pre-phasing of the strengths,
+// per-target deconvolution factors and post-phasing.  Replaces the type-3
+// stages of cuFINUFFT's setpts/execute (called at lib/kernels.cc.cu:66-75;
+// source absent from the reference).  The method is the published FINUFFT
+// type-3 scheme: with source centre C and target centre D per dimension,
+//   f_k = e^{i iflag s_k.C} / prod_d psi_hat(h_d g_d (s_dk - D_d))
+//         * sum_l b_l e^{i iflag h g (s_k - D) . l},
+//   b   = spread of c_j e^{i iflag D.(x_j - C)} at x'_j = (x_j - C) / g.
+#include <cfloat>
+
+#include "jfa_device.hpp"
+
+namespace jfa {
+namespace {
+
+struct V3 {
+    double v[3];
+};
+
+constexpr int kMinmaxBlocks = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(256) minmax_partial_kernel(int64_t n, const T *__restrict__ a,
+                                                             double *part) {
+    __shared__ double smin[256], smax[256];
+    double mn = DBL_MAX, mx = -DBL_MAX;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        double v = (double)a[i];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + off]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = smin[0];
+        part[2 * blockIdx.x + 1] = smax[0];
+    }
+}
+
+__global__ void __launch_bounds__(256) minmax_final_kernel(int nparts, const double *part,
+                                                           double *out) {
+    __shared__ double smin[256], smax[256];
+    double mn = DBL_MAX, mx = -DBL_MAX;
+    for (int i = threadIdx.x; i < nparts; i += 256) {
+        mn = fmin(mn, part[2 * i]);
+        mx = fmax(mx, part[2 * i + 1]);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + off]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smin[0];
+        out[1] = smax[0];
+    }
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+    t3_prephase_kernel(int64_t M, int ntr, const T *__restrict__ x, const T *__restrict__ y,
+                       const T *__restrict__ z, V3 C, V3 D, int iflag, const T *__restrict__ c,
+                       T *__restrict__ cp) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    double ph = D.v[0] * ((double)x[j] - C.v[0]);
+    if constexpr (DIM > 1) ph += D.v[1] * ((double)y[j] - C.v[1]);
+    if constexpr (DIM > 2) ph += D.v[2] * ((double)z[j] - C.v[2]);
+    double sn, cs;
+    sincos((double)iflag * ph, &sn, &cs);
+    for (int t = 0; t < ntr; ++t) {
+        const int64_t q = (int64_t)t * M + j;
+        const double a = (double)c[2 * q], b = (double)c[2 * q + 1];
+        cp[2 * q] = (T)(a * cs - b * sn);
+        cp[2 * q + 1] = (T)(a * sn + b * cs);
+    }
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+    t3_deconv_kernel(int64_t N, const T *__restrict__ s, const T *__restrict__ t,
+                     const T *__restrict__ u, V3 D, V3 hg, int ns, int nq,
+                     const double *__restrict__ quad, T *__restrict__ dec) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= N) return;
+    const T *ss[3] = {s, t, u};
+    double prod = 1.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double w = hg.v[d] * ((double)ss[d][k] - D.v[d]);
+        const double a = 0.5 * w * ns;
+        double acc = 0.0;
+        for (int q = 0; q < nq; ++q) acc += quad[q] * cos(a * quad[nq + q]);
+        prod *= ns * acc;
+    }
+    dec[k] = (T)(1.0 / prod);
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+    t3_postphase_kernel(int64_t N, int ntr, const T *__restrict__ s, const T *__restrict__ t,
+                        const T *__restrict__ u, V3 C, int iflag, const T *__restrict__ dec,
+                        T *__restrict__ f) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= N) return;
+    double ph = (double)s[k] * C.v[0];
+    if constexpr (DIM > 1) ph += (double)t[k] * C.v[1];
+    if constexpr (DIM > 2) ph += (double)u[k] * C.v[2];
+    double sn, cs;
+    sincos((double)iflag * ph, &sn, &cs);
+    const double sc = (double)dec[k];
+    cs *= sc;
+    sn *= sc;
+    for (int tr = 0; tr < ntr; ++tr) {
+        const int64_t q = (int64_t)tr * N + k;
+        const double a = (double)f[2 * q], b = (double)f[2 * q + 1];
+        f[2 * q] = (T)(a * cs - b * sn);
+        f[2 * q + 1] = (T)(a * sn + b * cs);
+    }
+}
+
+V3 v3(const double *a, int dim) {
+    V3 r{{0, 0, 0}};
+    for (int d = 0; d < dim; ++d) r.v[d] = a[d];
+    return r;
+}
+
+}  // namespace
+
+template <typename T>
+int launch_minmax(int64_t n, const T *a, double *out2, hipStream_t s) {
+    // out2 points at [min, max]; the partials live right after it
+    double *part = out2 + 2;
+    int64_t nb = (n + 255) / 256;
+    int blocks = (int)(nb < kMinmaxBlocks ? (nb > 0 ? nb : 1) : kMinmaxBlocks);
+    minmax_partial_kernel<T><<<blocks, 256, 0, s>>>(n, a, part);
+    minmax_final_kernel<<<1, 256, 0, s>>>(blocks, part, out2);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
+int launch_t3_prephase(int dim, int64_t M, int ntr, const T *x, const T *y, const T *z,
+                       const double *C, const double *D, int iflag, const T *c, T *cp,
+                       hipStream_t stream) {
+    if (M == 0 || ntr == 0) return JFA_OK;
+    unsigned blocks = (unsigned)((M + 255) / 256);
+    JFA_DISPATCH_DIM(dim, t3_prephase_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
+                              M, ntr, x, y, z, v3(C, dim), v3(D, dim), iflag, c, cp));
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
+int launch_t3_deconv_factors(int dim, int64_t N, const T *s, const T *t, const T *u,
+                             const double *D, const double *hg, int ns, int nq,
+                             const double *quad, T *dec, hipStream_t stream) {
+    if (N == 0) return JFA_OK;
+    unsigned blocks = (unsigned)((N + 255) / 256);
+    JFA_DISPATCH_DIM(dim, t3_deconv_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
+                              N, s, t, u, v3(D, dim), v3(hg, dim), ns, nq, quad, dec));
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
+int launch_t3_postphase(int dim, int64_t N, int ntr, const T *s, const T *t, const T *u,
+                        const double *C, int iflag, const T *dec, T *f, hipStream_t stream) {
+    if (N == 0 || ntr == 0) return JFA_OK;
+    unsigned blocks = (unsigned)((N + 255) / 256);
+    JFA_DISPATCH_DIM(dim, t3_postphase_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
+                              N, ntr, s, t, u, v3(C, dim), iflag, dec, f));
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+#define JFA_INST(T)                                                                            \
+    template int launch_minmax<T>(int64_t, const T *, double *, hipStream_t);                  \
+    template int launch_t3_prephase<T>(int, int64_t, int, const T *, const T *, const T *,     \
+                                       const double *, const double *, int, const T *, T *,   \
+                                       hipStream_t);                                           \
+    template int launch_t3_deconv_factors<T>(int, int64_t, const T *, const T *, const T *,    \
+                                             const double *, const double *, int, int,         \
+                                             const double *, T *, hipStream_t);                \
+    template int launch_t3_postphase<T>(int, int64_t, int, const T *, const T *, const T *,    \
+                                        const double *, int, const T *, T *, hipStream_t);
+JFA_INST(float)
+JFA_INST(double)
+#undef JFA_INST
+
+}  // namespace jfa

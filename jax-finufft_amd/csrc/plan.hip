@@ -1,0 +1,642 @@
+// Plan interface of the MI355X NUFFT backend: makeplan / setpts / execute /
+// destroy.  Replaces the cuFINUFFT plan the reference builds through
+// lib/cufinufft_wrapper.cc:48-99 (makeplan/setpts/execute/destroy shims) and
+// drives from run_nufft (lib/kernels.cc.cu:25-92).
+//
+// Differences by design:
+//  * all work is queued on the plan's stream; the host never waits (the
+//    reference synchronises at lib/kernels.cc.cu:83);
+//  * scratch is owned by the plan and grown only when a larger point count
+//    arrives, so a cached plan re-used across calls allocates nothing;
+//  * rocFFT plans are made once per batch size and reused.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "jfa_internal.hpp"
+#include "plan_state.hpp"
+
+using namespace jfa;
+
+namespace {
+
+std::once_flag g_rocfft_once;
+int g_rocfft_status = 0;
+
+int ensure_rocfft() {
+    std::call_once(g_rocfft_once, [] {
+        g_rocfft_status = rocfft_setup() == rocfft_status_success ? 0 : 1;
+    });
+    if (g_rocfft_status) {
+        set_error("rocfft_setup failed");
+        return JFA_ERR_FFT;
+    }
+    return JFA_OK;
+}
+
+template <typename T>
+int upload(const std::vector<double> &h, void **d) {
+    std::vector<T> tmp(h.begin(), h.end());
+    JFA_HIP_CHECK(hipMalloc(d, sizeof(T) * std::max<size_t>(tmp.size(), 1)));
+    JFA_HIP_CHECK(hipMemcpy(*d, tmp.data(), sizeof(T) * tmp.size(), hipMemcpyHostToDevice));
+    return JFA_OK;
+}
+
+void free_ptr(void *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Bin sizes: 2D 32x32 and 1D 1024 cells; 3D 16x16x8 (single) / 8x8x8
+// (double).  Each is clamped to half the fine grid so that a tile origin plus
+// its extent wraps at most once.
+void default_bins(jfa_plan p) {
+    int b[3] = {1, 1, 1};
+    if (p->dim == 1) b[0] = 1024;
+    if (p->dim == 2) b[0] = b[1] = 32;
+    if (p->dim == 3) {
+        if (p->is_double) b[0] = b[1] = b[2] = 8;
+        else { b[0] = b[1] = 16; b[2] = 8; }
+    }
+    const int o[3] = {p->opts.gpu_binsizex, p->opts.gpu_binsizey, p->opts.gpu_binsizez};
+    GridGeom &g = p->geom;
+    g.nbins = 1;
+    for (int d = 0; d < 3; ++d) {
+        int bs = d < p->dim ? (o[d] > 0 ? o[d] : b[d]) : 1;
+        if (d < p->dim) {
+            int64_t cap = std::max<int64_t>(1, g.nf[d] / 2);
+            if (bs > cap) bs = (int)cap;
+        }
+        g.bs[d] = bs;
+        g.nb[d] = (int)((g.nf[d] + bs - 1) / bs);
+        g.nbins *= g.nb[d];
+    }
+}
+
+size_t tile_lds_bytes(jfa_plan p) {
+    size_t cells = 1;
+    for (int d = 0; d < p->dim; ++d) cells *= (size_t)(p->geom.bs[d] + p->geom.ns - 1);
+    return cells * 2 * (p->is_double ? 8 : 4);
+}
+
+int resolve_method(jfa_plan p) {
+    const size_t kMaxTileLds = 64 * 1024;
+    int m = p->opts.gpu_method;
+    if (m == JFA_METHOD_AUTO) m = tile_lds_bytes(p) <= kMaxTileLds ? JFA_METHOD_SM : JFA_METHOD_GM_SORT;
+    // an explicit LDS request with a wide kernel: halve the largest bin until
+    // the tile fits
+    while (m == JFA_METHOD_SM && tile_lds_bytes(p) > kMaxTileLds) {
+        GridGeom &g = p->geom;
+        int dmax = 0;
+        for (int d = 1; d < p->dim; ++d)
+            if (g.bs[d] > g.bs[dmax]) dmax = d;
+        if (g.bs[dmax] <= 1) break;
+        g.bs[dmax] = (g.bs[dmax] + 1) / 2;
+        g.nbins = 1;
+        for (int d = 0; d < 3; ++d) {
+            g.nb[d] = (int)((g.nf[d] + g.bs[d] - 1) / g.bs[d]);
+            g.nbins *= g.nb[d];
+        }
+    }
+    if (m == JFA_METHOD_SM && tile_lds_bytes(p) > kMaxTileLds) {
+        set_error("LDS subproblem tile of %zu bytes exceeds 64 KiB; use smaller bins or "
+                  "gpu_method=1", tile_lds_bytes(p));
+        return -1;
+    }
+    if (m != JFA_METHOD_SM && m != JFA_METHOD_GM_SORT) {
+        set_error("gpu_method must be 0, 1 or 2 (got %d)", m);
+        return -1;
+    }
+    return m;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    bool changed = false;
+    int set(int dev) {
+        if (dev < 0) return JFA_OK;
+        JFA_HIP_CHECK(hipGetDevice(&prev));
+        if (prev != dev) {
+            JFA_HIP_CHECK(hipSetDevice(dev));
+            changed = true;
+        }
+        return JFA_OK;
+    }
+    ~DeviceGuard() {
+        if (changed) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ fft
+
+int jfa::fft_exec(jfa_plan p, void *data, int nbatch) {
+    if (nbatch <= 0) return JFA_OK;
+    rocfft_plan plan = nullptr;
+    auto it = p->fft_plans.find(nbatch);
+    if (it == p->fft_plans.end()) {
+        size_t lengths[3];
+        for (int d = 0; d < p->dim; ++d) lengths[d] = (size_t)p->geom.nf[d];
+        rocfft_status st = rocfft_plan_create(
+            &plan, rocfft_placement_inplace,
+            p->iflag > 0 ? rocfft_transform_type_complex_inverse
+                         : rocfft_transform_type_complex_forward,
+            p->is_double ? rocfft_precision_double : rocfft_precision_single, (size_t)p->dim,
+            lengths, (size_t)nbatch, nullptr);
+        if (st != rocfft_status_success) {
+            set_error("rocfft_plan_create failed (status %d)", (int)st);
+            return JFA_ERR_FFT;
+        }
+        size_t ws = 0;
+        rocfft_plan_get_work_buffer_size(plan, &ws);
+        if (ws > p->fft_work_bytes) {
+            free_ptr(p->d_fft_work);
+            JFA_HIP_CHECK(hipMalloc(&p->d_fft_work, ws));
+            p->fft_work_bytes = ws;
+        }
+        p->fft_plans[nbatch] = plan;
+    } else {
+        plan = it->second;
+    }
+    if (!p->fft_info) {
+        if (rocfft_execution_info_create(&p->fft_info) != rocfft_status_success) {
+            set_error("rocfft_execution_info_create failed");
+            return JFA_ERR_FFT;
+        }
+    }
+    rocfft_execution_info_set_stream(p->fft_info, p->stream);
+    if (p->fft_work_bytes)
+        rocfft_execution_info_set_work_buffer(p->fft_info, p->d_fft_work, p->fft_work_bytes);
+    void *bufs[1] = {data};
+    rocfft_status st = rocfft_execute(plan, bufs, nullptr, p->fft_info);
+    if (st != rocfft_status_success) {
+        set_error("rocfft_execute failed (status %d)", (int)st);
+        return JFA_ERR_FFT;
+    }
+    return JFA_OK;
+}
+
+// ------------------------------------------------------------------ timing
+
+namespace jfa {
+struct StageTimer {
+    jfa_plan p;
+    bool on;
+    std::vector<std::pair<const char *, hipEvent_t>> marks;
+    explicit StageTimer(jfa_plan pl) : p(pl), on(pl->opts.debug > 0) { mark("start"); }
+    void mark(const char *name) {
+        if (!on) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, p->stream);
+        marks.push_back({name, e});
+    }
+    void report(const char *what) {
+        if (!on || marks.size() < 2) return;
+        (void)hipEventSynchronize(marks.back().second);
+        fprintf(stderr, "[jfa %s] type %d, %dd, ntransf=%d, M=%lld:", what, p->type, p->dim,
+                p->ntransf, (long long)p->M);
+        for (size_t i = 1; i < marks.size(); ++i) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second);
+            fprintf(stderr, " %s %.4f ms", marks[i].first, ms);
+        }
+        fprintf(stderr, "\n");
+        for (auto &m : marks) (void)hipEventDestroy(m.second);
+        marks.clear();
+    }
+};
+}  // namespace jfa
+
+// ------------------------------------------------------------------ plan API
+
+extern "C" void jfa_default_opts(jfa_opts *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->modeord = 0;
+    o->upsampfac = 2.0;
+    o->gpu_method = JFA_METHOD_AUTO;
+    o->gpu_sort = 1;
+    o->gpu_kerevalmeth = 1;
+    o->gpu_maxbatchsize = 0;
+    o->debug = 0;
+    o->gpu_maxsubprobsize = 0;
+    o->gpu_spreadinterponly = 0;
+    o->gpu_device_id = -1;
+    o->gpu_stream = nullptr;
+    o->width_rule = 0;
+    o->kernel_width = 0;
+}
+
+extern "C" const char *jfa_last_error(void) { return jfa::last_error(); }
+
+// Fill the kernel tables (Horner coefficients and per-dimension 1/psi_hat) for
+// a grid geometry that is already set.
+int jfa::upload_kernel_tables(jfa_plan p, const int64_t *nmodes_for_dk) {
+    const KernelParams &kp = p->kp;
+    std::vector<double> coef((size_t)kp.nc * kp.ns);
+    horner_fit(kp.ns, kp.nc, kp.beta, coef.data());
+    free_ptr(p->d_coef);
+    int rc = p->is_double ? upload<double>(coef, &p->d_coef) : upload<float>(coef, &p->d_coef);
+    if (rc) return rc;
+    for (int d = 0; d < 3; ++d) {
+        free_ptr(p->d_dk[d]);
+        int64_t n = (d < p->dim && nmodes_for_dk) ? nmodes_for_dk[d] : 1;
+        std::vector<double> ph((size_t)n, 1.0);
+        if (d < p->dim && nmodes_for_dk) {
+            kernel_fourier(kp.ns, kp.beta, (double)p->geom.nf[d], (double)(-(n / 2)), n, ph.data());
+            for (auto &v : ph) v = 1.0 / v;
+        }
+        rc = p->is_double ? upload<double>(ph, &p->d_dk[d]) : upload<float>(ph, &p->d_dk[d]);
+        if (rc) return rc;
+    }
+    return JFA_OK;
+}
+
+extern "C" int jfa_makeplan(int type, int dim, const int64_t *nmodes, int iflag, int ntransf,
+                            double eps, int is_double, const jfa_opts *opts,
+                            jfa_plan *plan_out) {
+    if (!plan_out) {
+        set_error("plan_out is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    *plan_out = nullptr;
+    if (type < 1 || type > 3) {
+        set_error("type must be 1, 2 or 3 (got %d)", type);
+        return JFA_ERR_BAD_ARG;
+    }
+    if (dim < 1 || dim > 3) {
+        set_error("dim must be 1, 2 or 3 (got %d)", dim);
+        return JFA_ERR_BAD_ARG;
+    }
+    if (ntransf < 1) {
+        set_error("ntransf must be >= 1 (got %d)", ntransf);
+        return JFA_ERR_BAD_ARG;
+    }
+    if (!(eps > 0)) {
+        set_error("eps must be positive");
+        return JFA_ERR_BAD_ARG;
+    }
+    if (int rc = ensure_rocfft()) return rc;
+
+    jfa_opts o;
+    if (opts) o = *opts;
+    else jfa_default_opts(&o);
+    if (!(o.upsampfac >= 1.0)) {
+        set_error("upsampfac must be >= 1 (got %g)", o.upsampfac);
+        return JFA_ERR_BAD_ARG;
+    }
+    if (o.kernel_width != 0 && (o.kernel_width < 2 || o.kernel_width > kMaxNs)) {
+        set_error("kernel_width must be in [2, %d]", kMaxNs);
+        return JFA_ERR_BAD_ARG;
+    }
+    if (o.modeord != 0 && o.modeord != 1) {
+        set_error("modeord must be 0 or 1");
+        return JFA_ERR_BAD_ARG;
+    }
+
+    DeviceGuard dg;
+    if (int rc = dg.set(o.gpu_device_id)) return rc;
+
+    auto *p = new jfa_plan_s();
+    p->type = type;
+    p->dim = dim;
+    p->is_double = is_double ? 1 : 0;
+    p->iflag = iflag >= 0 ? 1 : -1;
+    p->ntransf = ntransf;
+    p->eps = eps;
+    p->opts = o;
+    p->stream = (hipStream_t)o.gpu_stream;
+    if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
+    p->batch = o.gpu_maxbatchsize > 0 ? std::min(o.gpu_maxbatchsize, ntransf) : ntransf;
+    p->kp = choose_kernel(eps, o.upsampfac, o.width_rule, p->is_double, o.kernel_width);
+    p->maxsub = o.gpu_maxsubprobsize > 0 ? o.gpu_maxsubprobsize : default_maxsub(p);
+
+    GridGeom &g = p->geom;
+    g.dim = dim;
+    g.ns = p->kp.ns;
+    for (int d = 0; d < 3; ++d) {
+        p->nmodes[d] = 1;
+        g.nf[d] = 1;
+        g.scale[d] = 1.0;
+        g.shift[d] = 0.0;
+    }
+    if (type != 3) {
+        for (int d = 0; d < dim; ++d) {
+            if (!nmodes || nmodes[d] < 1) {
+                jfa_destroy(p);
+                set_error("nmodes[%d] must be >= 1", d);
+                return JFA_ERR_BAD_ARG;
+            }
+            p->nmodes[d] = nmodes[d];
+            if (o.gpu_spreadinterponly) {
+                if (nmodes[d] < 2 * p->kp.ns) {
+                    jfa_destroy(p);
+                    set_error("spread/interp-only grid %lld is smaller than 2*width %d",
+                              (long long)nmodes[d], 2 * p->kp.ns);
+                    return JFA_ERR_BAD_ARG;
+                }
+                g.nf[d] = nmodes[d];
+            } else {
+                g.nf[d] = fine_size(nmodes[d], o.upsampfac, p->kp.ns);
+            }
+            g.scale[d] = (double)g.nf[d] / (2.0 * M_PI);
+        }
+        p->Ntot = p->nmodes[0] * p->nmodes[1] * p->nmodes[2];
+        p->F = g.nf[0] * g.nf[1] * g.nf[2];
+        default_bins(p);
+        p->method = resolve_method(p);
+        if (p->method < 0) {
+            jfa_destroy(p);
+            return JFA_ERR_BAD_ARG;
+        }
+        int rc = upload_kernel_tables(p, p->nmodes);
+        if (rc) {
+            jfa_destroy(p);
+            return rc;
+        }
+        if (!o.gpu_spreadinterponly) {
+            const size_t bytes = (size_t)p->batch * p->F * 2 * (p->is_double ? 8 : 4);
+            if (hipMalloc(&p->d_fw, bytes) != hipSuccess) {
+                jfa_destroy(p);
+                set_error("could not allocate the %zu-byte fine grid", bytes);
+                return JFA_ERR_ALLOC;
+            }
+        }
+    } else {
+        p->Ntot = 0;
+        p->F = 0;
+        int rc = jfa::type3_init(p);
+        if (rc) {
+            jfa_destroy(p);
+            return rc;
+        }
+    }
+    *plan_out = p;
+    return p->kp.warn ? JFA_WARN_EPS_TOO_SMALL : JFA_OK;
+}
+
+int jfa::default_maxsub(jfa_plan p) {
+    // One subproblem per bin unless a bin is crowded: 4x the mean points per
+    // bin of a uniformly filled grid, at least 1024.
+    (void)p;
+    return 4096;
+}
+
+// Grow the point-dependent scratch to hold M points.
+int jfa::reserve_points(jfa_plan p, int64_t M) {
+    const size_t real = p->is_double ? 8 : 4;
+    if (p->geom.nbins + 1 > p->cap_bins) {
+        for (int **q : {&p->d_bin_count, &p->d_bin_start, &p->d_sub_start})
+            free_ptr(reinterpret_cast<void *&>(*q));
+        const size_t nb = (size_t)p->geom.nbins + 1;
+        // bin_count is padded to a multiple of 4 ints for the vector fill
+        JFA_HIP_CHECK(hipMalloc(&p->d_bin_count, sizeof(int) * (nb + 4)));
+        JFA_HIP_CHECK(hipMalloc(&p->d_bin_start, sizeof(int) * nb));
+        JFA_HIP_CHECK(hipMalloc(&p->d_sub_start, sizeof(int) * nb));
+        if (!p->d_nsub) JFA_HIP_CHECK(hipMalloc(&p->d_nsub, sizeof(int) * 4));
+        p->cap_bins = (int64_t)nb;
+    }
+    if (M > p->cap_M) {
+        for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_bin_of,
+                         (void **)&p->d_rank_of, &p->d_t})
+            free_ptr(*q);
+        const int64_t cap = std::max<int64_t>(M, 1);
+        JFA_HIP_CHECK(hipMalloc(&p->d_idx, sizeof(int) * cap));
+        JFA_HIP_CHECK(hipMalloc(&p->d_l0, sizeof(int) * cap * p->dim));
+        JFA_HIP_CHECK(hipMalloc(&p->d_bin_of, sizeof(int) * cap));
+        JFA_HIP_CHECK(hipMalloc(&p->d_rank_of, sizeof(int) * cap));
+        JFA_HIP_CHECK(hipMalloc(&p->d_t, real * cap * p->dim));
+        p->cap_M = cap;
+    }
+    return JFA_OK;
+}
+
+template <typename T>
+static int sort_points(jfa_plan p, int64_t M, const T *x, const T *y, const T *z) {
+    const GridGeom &g = p->geom;
+    const int sorted = (p->method == JFA_METHOD_SM || p->opts.gpu_sort) ? 1 : 0;
+    int rc;
+    if (sorted) {
+        if ((rc = launch_fill_zero(p->d_bin_count, sizeof(int) * (g.nbins + 1), p->stream)))
+            return rc;
+        if ((rc = launch_fold_bin<T>(g, M, x, y, z, p->d_bin_of, p->d_rank_of, p->d_bin_count,
+                                     p->stream)))
+            return rc;
+        if ((rc = launch_bin_scan(g, p->d_bin_count, p->d_bin_start, p->d_sub_start, p->d_nsub,
+                                  p->maxsub, p->stream)))
+            return rc;
+    }
+    return launch_scatter<T>(g, M, x, y, z, p->d_bin_of, p->d_rank_of, p->d_bin_start, sorted,
+                             p->d_idx, p->d_l0, (T *)p->d_t, p->stream);
+}
+
+int jfa::setpts_12(jfa_plan p, int64_t M, const void *x, const void *y, const void *z) {
+    if (M < 0 || M > INT_MAX) {
+        set_error("number of points %lld outside [0, 2^31)", (long long)M);
+        return JFA_ERR_TOO_LARGE;
+    }
+    const void *xs[3] = {x, y, z};
+    for (int d = 0; d < p->dim; ++d)
+        if (M > 0 && !xs[d]) {
+            set_error("points for dimension %d are NULL", d);
+            return JFA_ERR_BAD_ARG;
+        }
+    int rc = reserve_points(p, M);
+    if (rc) return rc;
+    p->M = M;
+    if (M == 0) return JFA_OK;
+    if (p->is_double)
+        return sort_points<double>(p, M, (const double *)x, (const double *)y,
+                                   (const double *)z);
+    return sort_points<float>(p, M, (const float *)x, (const float *)y, (const float *)z);
+}
+
+extern "C" int jfa_setpts(jfa_plan p, int64_t M, const void *x, const void *y, const void *z,
+                          int64_t N, const void *s, const void *t, const void *u) {
+    if (!p) {
+        set_error("plan is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    DeviceGuard dg;
+    if (int rc = dg.set(p->device)) return rc;
+    p->used = true;
+    StageTimer tm(p);
+    int rc = p->type == 3 ? jfa::type3_setpts(p, M, x, y, z, N, s, t, u)
+                          : jfa::setpts_12(p, M, x, y, z);
+    tm.mark("setpts");
+    tm.report("setpts");
+    return rc;
+}
+
+template <typename T>
+SortedPts<T> jfa::sorted_view(jfa_plan p) {
+    SortedPts<T> s;
+    s.M = p->M;
+    s.idx = p->d_idx;
+    for (int d = 0; d < 3; ++d) {
+        s.l0[d] = d < p->dim ? p->d_l0 + d * p->M : nullptr;
+        s.t[d] = d < p->dim ? (const T *)p->d_t + d * p->M : nullptr;
+    }
+    return s;
+}
+template SortedPts<float> jfa::sorted_view<float>(jfa_plan);
+template SortedPts<double> jfa::sorted_view<double>(jfa_plan);
+
+BinIndex jfa::bin_view(jfa_plan p) {
+    BinIndex b;
+    b.bin_start = p->d_bin_start;
+    b.sub_start = p->d_sub_start;
+    b.nsub = p->d_nsub;
+    b.maxsub = p->maxsub;
+    return b;
+}
+
+// Type 1 / type 2 execution on the plan's current points.  c has ntransf*M
+// complex values, fk ntransf*Ntot.
+template <typename T>
+int jfa::execute_12(jfa_plan p, T *c, T *fk) {
+    const GridGeom &g = p->geom;
+    const SortedPts<T> pts = sorted_view<T>(p);
+    const BinIndex bi = bin_view(p);
+    const T *coef = (const T *)p->d_coef;
+    const T *dk[3] = {(const T *)p->d_dk[0], (const T *)p->d_dk[1], (const T *)p->d_dk[2]};
+    const int kem = p->opts.gpu_kerevalmeth ? 1 : 0;
+    const int64_t M = p->M;
+    StageTimer tm(p);
+    int rc = JFA_OK;
+    if (p->opts.gpu_spreadinterponly) {
+        // the mode array is the fine grid itself
+        if (p->type == 1) {
+            if ((rc = launch_fill_zero(fk, sizeof(T) * 2 * p->F * p->ntransf, p->stream)))
+                return rc;
+            rc = launch_spread<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, c, M, fk, p->F,
+                                  p->ntransf, p->stream);
+            tm.mark("spread");
+        } else {
+            rc = launch_interp<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, fk, p->F, c, M,
+                                  p->ntransf, p->stream);
+            tm.mark("interp");
+        }
+        tm.report("execute");
+        return rc;
+    }
+    T *fw = (T *)p->d_fw;
+    for (int t0 = 0; t0 < p->ntransf; t0 += p->batch) {
+        const int nb = std::min(p->batch, p->ntransf - t0);
+        T *cb = c + 2 * (int64_t)t0 * M;
+        T *fb = fk + 2 * (int64_t)t0 * p->Ntot;
+        if (p->type == 1) {
+            if ((rc = launch_fill_zero(fw, sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
+            if ((rc = launch_spread<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, cb, M, fw,
+                                       p->F, nb, p->stream)))
+                return rc;
+            tm.mark("spread");
+            if ((rc = fft_exec(p, fw, nb))) return rc;
+            tm.mark("fft");
+            if ((rc = launch_deconv<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2],
+                                       p->opts.modeord, fw, p->F, fb, p->Ntot, nb, p->stream)))
+                return rc;
+            tm.mark("deconv");
+        } else {
+            if ((rc = launch_amplify<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2],
+                                        p->opts.modeord, fb, p->Ntot, fw, p->F, nb, p->stream)))
+                return rc;
+            tm.mark("amplify");
+            if ((rc = fft_exec(p, fw, nb))) return rc;
+            tm.mark("fft");
+            if ((rc = launch_interp<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, fw, p->F, cb,
+                                       M, nb, p->stream)))
+                return rc;
+            tm.mark("interp");
+        }
+    }
+    tm.report("execute");
+    return rc;
+}
+template int jfa::execute_12<float>(jfa_plan, float *, float *);
+template int jfa::execute_12<double>(jfa_plan, double *, double *);
+
+extern "C" int jfa_execute(jfa_plan p, void *c, void *fk) {
+    if (!p) {
+        set_error("plan is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    DeviceGuard dg;
+    if (int rc = dg.set(p->device)) return rc;
+    p->used = true;
+    if (p->type == 3) return jfa::type3_execute(p, c, fk);
+    if ((p->M > 0 && !c) || (p->Ntot > 0 && !fk)) {
+        set_error("execute: NULL data pointer");
+        return JFA_ERR_BAD_ARG;
+    }
+    if (p->M == 0) {
+        // no points: type 1 output is zero, type 2 has nothing to write
+        if (p->type == 1)
+            return launch_fill_zero(fk, (int64_t)(p->is_double ? 16 : 8) * p->Ntot * p->ntransf,
+                                    p->stream);
+        return JFA_OK;
+    }
+    if (p->is_double) return execute_12<double>(p, (double *)c, (double *)fk);
+    return execute_12<float>(p, (float *)c, (float *)fk);
+}
+
+extern "C" int jfa_set_stream(jfa_plan p, jfa_stream_t stream) {
+    if (!p) {
+        set_error("plan is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (s == p->stream) return JFA_OK;
+    DeviceGuard dg;
+    if (int rc = dg.set(p->device)) return rc;
+    if (p->used) {
+        if (!p->order_event)
+            JFA_HIP_CHECK(hipEventCreateWithFlags(&p->order_event, hipEventDisableTiming));
+        JFA_HIP_CHECK(hipEventRecord(p->order_event, p->stream));
+        JFA_HIP_CHECK(hipStreamWaitEvent(s, p->order_event, 0));
+    }
+    p->stream = s;
+    if (p->inner) return jfa_set_stream(p->inner, stream);
+    return JFA_OK;
+}
+
+extern "C" int jfa_destroy(jfa_plan p) {
+    if (!p) return JFA_OK;
+    DeviceGuard dg;
+    (void)dg.set(p->device);
+    if (p->used) (void)hipStreamSynchronize(p->stream);
+    if (p->inner) jfa_destroy(p->inner);
+    for (auto &kv : p->fft_plans) rocfft_plan_destroy(kv.second);
+    if (p->fft_info) rocfft_execution_info_destroy(p->fft_info);
+    free_ptr(p->d_fft_work);
+    free_ptr(p->d_coef);
+    for (auto &d : p->d_dk) free_ptr(d);
+    free_ptr(p->d_fw);
+    for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_bin_of,
+                     (void **)&p->d_rank_of, &p->d_t, (void **)&p->d_bin_count,
+                     (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub})
+        free_ptr(*q);
+    jfa::type3_free(p);
+    if (p->order_event) (void)hipEventDestroy(p->order_event);
+    delete p;
+    return JFA_OK;
+}
+
+extern "C" int jfa_plan_info(jfa_plan p, int *ns, double *beta, int64_t *nf) {
+    if (!p) {
+        set_error("plan is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    if (ns) *ns = p->kp.ns;
+    if (beta) *beta = p->kp.beta;
+    if (nf)
+        for (int d = 0; d < 3; ++d) nf[d] = p->geom.nf[d];
+    return JFA_OK;
+}

@@ -1,0 +1,75 @@
+// The plan object behind the opaque jfa_plan handle.
+#pragma once
+
+#include <map>
+
+#include "jfa_internal.hpp"
+
+struct jfa_plan_s {
+    int type = 1, dim = 1, is_double = 0, iflag = 1, ntransf = 1, batch = 1;
+    double eps = 1e-6;
+    int64_t nmodes[3] = {1, 1, 1};  // x fastest; 1 for unused dims
+    int64_t Ntot = 1;               // modes per transform (targets for type 3)
+    int64_t F = 1;                  // fine-grid cells per transform
+    jfa_opts opts{};
+    jfa::KernelParams kp;
+    jfa::GridGeom geom{};
+    int method = JFA_METHOD_SM;
+    int maxsub = 4096;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool used = false;                 // work has been queued on `stream`
+    hipEvent_t order_event = nullptr;  // orders work across jfa_set_stream
+
+    // kernel tables
+    void *d_coef = nullptr;            // Horner coefficients (nc x ns)
+    void *d_dk[3] = {nullptr, nullptr, nullptr};  // 1 / psi_hat per dim
+    void *d_fw = nullptr;              // batch x F complex fine grid
+
+    // rocFFT
+    std::map<int, rocfft_plan> fft_plans;  // by batch count
+    rocfft_execution_info fft_info = nullptr;
+    void *d_fft_work = nullptr;
+    size_t fft_work_bytes = 0;
+
+    // sorted points
+    int64_t M = 0, cap_M = 0, cap_bins = 0;
+    int *d_idx = nullptr, *d_l0 = nullptr, *d_bin_of = nullptr, *d_rank_of = nullptr;
+    void *d_t = nullptr;
+    int *d_bin_count = nullptr, *d_bin_start = nullptr, *d_sub_start = nullptr, *d_nsub = nullptr;
+
+    // type 3
+    jfa_plan inner = nullptr;     // type-2 plan from the spread grid to the targets
+    const void *t3_x[3] = {nullptr, nullptr, nullptr};   // user source coordinates
+    const void *t3_s[3] = {nullptr, nullptr, nullptr};   // user target coordinates
+    double t3_C[3] = {0, 0, 0}, t3_D[3] = {0, 0, 0};      // source / target centres
+    double t3_gam[3] = {1, 1, 1}, t3_h[3] = {1, 1, 1};
+    int64_t t3_nf_cap = 0;        // fine-grid cells allocated for d_fw
+    void *d_cp = nullptr;         // prephased strengths, batch x M complex
+    int64_t cap_cp = 0;
+    void *d_t3_deconv = nullptr;  // per-target 1 / prod psi_hat (real)
+    int64_t cap_t3_deconv = 0;
+    double *d_minmax = nullptr;   // scratch for extents (6 x [min, max])
+    void *d_quad = nullptr;       // quadrature table for psi_hat at arbitrary omega
+};
+
+namespace jfa {
+
+int fft_exec(jfa_plan p, void *data, int nbatch);
+int upload_kernel_tables(jfa_plan p, const int64_t *nmodes_for_dk);
+int default_maxsub(jfa_plan p);
+int reserve_points(jfa_plan p, int64_t M);
+int setpts_12(jfa_plan p, int64_t M, const void *x, const void *y, const void *z);
+template <typename T>
+SortedPts<T> sorted_view(jfa_plan p);
+BinIndex bin_view(jfa_plan p);
+template <typename T>
+int execute_12(jfa_plan p, T *c, T *fk);
+
+int type3_init(jfa_plan p);
+int type3_setpts(jfa_plan p, int64_t M, const void *x, const void *y, const void *z, int64_t N,
+                 const void *s, const void *t, const void *u);
+int type3_execute(jfa_plan p, void *c, void *fk);
+void type3_free(jfa_plan p);
+
+}  // namespace jfa

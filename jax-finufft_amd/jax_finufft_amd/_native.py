@@ -1,0 +1,280 @@
+"""ctypes binding of the C-ABI library ``libjfa.so`` and the call-site logic
+that the reference keeps in its lowering.
+
+``lowering`` mirrors src/jax_finufft/lowering.py:28-178: it picks the entry
+point ``nufft{ndim}d{type}{f}``, reverses the point arrays and mode extents into
+the backend's x-fastest ("Fortran") order (:266-275), resolves options with
+``unpack_opts(opts, type, True)`` (:277-280) and passes the same fifteen
+attributes the reference's GPU handlers take (:327-344).  Instead of emitting
+an XLA custom call it calls the entry point directly on torch device tensors,
+on torch's current stream (the stream XLA would hand the handler).
+
+There is no CPU fallback: if ``libjfa.so`` is missing or no HIP device is
+visible, calls raise.
+"""
+
+import ctypes
+import os
+import threading
+
+import torch
+
+from . import options
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjfa.so")
+
+_lib = None
+_lock = threading.Lock()
+
+c_int, c_int64, c_double, c_float, c_void_p = (
+    ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_float, ctypes.c_void_p,
+)
+
+
+class JfaOpts(ctypes.Structure):
+    """Mirror of ``jfa_opts`` in include/jax_finufft_amd.h."""
+
+    _fields_ = [
+        ("modeord", c_int),
+        ("upsampfac", c_double),
+        ("gpu_method", c_int),
+        ("gpu_sort", c_int),
+        ("gpu_kerevalmeth", c_int),
+        ("gpu_maxbatchsize", c_int),
+        ("debug", c_int),
+        ("gpu_binsizex", c_int),
+        ("gpu_binsizey", c_int),
+        ("gpu_binsizez", c_int),
+        ("gpu_maxsubprobsize", c_int),
+        ("gpu_spreadinterponly", c_int),
+        ("gpu_device_id", c_int),
+        ("gpu_stream", c_void_p),
+        ("width_rule", c_int),
+        ("kernel_width", c_int),
+    ]
+
+
+ENTRY_POINTS = [
+    f"jfa_nufft{d}d{t}{s}" for t in (1, 2, 3) for d in (1, 2, 3) for s in ("f", "")
+]
+PLAN_API = [
+    "jfa_default_opts", "jfa_makeplan", "jfa_setpts", "jfa_execute", "jfa_set_stream",
+    "jfa_destroy", "jfa_plan_info", "jfa_last_error", "jfa_cache_size", "jfa_cache_clear",
+]
+
+
+class NufftError(RuntimeError):
+    """A non-zero (non-warning) return code from the backend."""
+
+
+def lib():
+    """Load ``libjfa.so`` once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"HIP extension not built ({LIB_PATH} missing): run "
+                "`make -C jax-finufft_amd/csrc` or __graft_entry__.build()"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        L.jfa_last_error.restype = ctypes.c_char_p
+        L.jfa_default_opts.argtypes = [ctypes.POINTER(JfaOpts)]
+        L.jfa_default_opts.restype = None
+        L.jfa_makeplan.argtypes = [
+            c_int, c_int, ctypes.POINTER(c_int64), c_int, c_int, c_double, c_int,
+            ctypes.POINTER(JfaOpts), ctypes.POINTER(c_void_p),
+        ]
+        L.jfa_setpts.argtypes = [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
+                                 c_void_p, c_void_p, c_void_p]
+        L.jfa_execute.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.jfa_set_stream.argtypes = [c_void_p, c_void_p]
+        L.jfa_destroy.argtypes = [c_void_p]
+        L.jfa_plan_info.argtypes = [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_double),
+                                    ctypes.POINTER(c_int64)]
+        L.jfa_cache_size.restype = c_int
+        L.jfa_cache_clear.restype = None
+        for name in ENTRY_POINTS:
+            fn = getattr(L, name)
+            single = name.endswith("f")
+            t = int(name[len("jfa_nufft") + 2])
+            d = int(name[len("jfa_nufft")])
+            nptr = 1 + d * (2 if t == 3 else 1) + 1
+            fn.argtypes = (
+                [c_void_p, c_float if single else c_double]
+                + [c_int64] * 8
+                + [c_double]
+                + [c_int64] * 5
+                + [c_void_p] * nptr
+            )
+            fn.restype = c_int
+        _lib = L
+        return _lib
+
+
+def last_error():
+    return lib().jfa_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc > 1:
+        raise NufftError(f"{what} failed with code {rc}: {last_error()}")
+    return rc
+
+
+def _stream_handle(device):
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _require_device(*tensors):
+    for t in tensors:
+        if not t.is_cuda:
+            raise ValueError(
+                "jax_finufft_amd runs on a HIP device: pass tensors on a 'cuda' device "
+                f"(got {t.device})"
+            )
+
+
+def lowering(source, *points, output_shape, iflag, eps, opts, nufft_type):
+    """Run one canonical-layout transform through the C-ABI entry point.
+
+    ``source`` is (n_tot, n_transf, M) for types 1/3 or (n_tot, n_transf, *N)
+    for type 2; ``points`` are (n_tot, M) (type 3: sources then targets, each
+    (n_tot, M) / (n_tot, N)).  Returns a new tensor in the canonical output
+    layout.
+    """
+    ndim = len(points) // 2 if nufft_type == 3 else len(points)
+    if not 1 <= ndim <= 3:
+        raise ValueError("Only 1-, 2-, and 3-dimensions are supported")
+    from .shapes import abstract_eval
+
+    out_shape, out_dtype = abstract_eval(
+        source, *points, output_shape=output_shape, nufft_type=nufft_type
+    )
+    _require_device(source, *points)
+    single = source.dtype == torch.complex64
+    suffix = "f" if single else ""
+    n_tot, n_transf = source.shape[0], source.shape[1]
+    n_j = points[0].shape[1]
+
+    if nufft_type == 1:
+        n_k = list(output_shape)
+    elif nufft_type == 2:
+        n_k = list(source.shape[2:])
+    n_k_full = [0, 0, 0]
+    if nufft_type != 3:
+        n_k_full[:ndim] = n_k[::-1]
+    else:
+        n_k_full[0] = points[ndim].shape[1]
+    points_fortran = list(points[:ndim][::-1]) + list(points[ndim:][::-1])
+
+    if opts is None:
+        opts = options.Opts()
+    opts = options.unpack_opts(opts, nufft_type, True)
+    assert isinstance(opts, options.Opts)
+    native = opts.to_cufinufft_opts()
+
+    source = source.contiguous()
+    points_fortran = [p.contiguous() for p in points_fortran]
+    out = torch.empty(out_shape, dtype=out_dtype, device=source.device)
+    if out.numel() == 0:
+        return out
+    fn = getattr(lib(), f"jfa_nufft{ndim}d{nufft_type}{suffix}")
+    with torch.cuda.device(source.device):
+        rc = fn(
+            _stream_handle(source.device),
+            float(eps),
+            int(iflag), int(n_tot), int(n_transf), int(n_j),
+            int(n_k_full[0]), int(n_k_full[1]), int(n_k_full[2]),
+            int(native.modeord), float(native.upsampfac), int(native.gpu_method),
+            int(native.gpu_sort), int(native.gpu_kerevalmeth), int(native.gpu_maxbatchsize),
+            int(native.debug),
+            c_void_p(source.data_ptr()),
+            *[c_void_p(p.data_ptr()) for p in points_fortran],
+            c_void_p(out.data_ptr()),
+        )
+    check(rc, f"nufft{ndim}d{nufft_type}{suffix}")
+    return out
+
+
+class Plan:
+    """The guru plan interface (makeplan / setpts / execute / destroy) on torch
+    device tensors, in the backend's native x-fastest axis order.
+
+    ``n_modes`` and the point arrays are given in *user* order (as for
+    ``nufft1``); they are reversed here exactly as the lowering does.
+    """
+
+    def __init__(self, nufft_type, n_modes, iflag=1, ntransf=1, eps=1e-6, dtype=torch.complex64,
+                 device=None, **native_opts):
+        self._plan = c_void_p(None)
+        L = lib()
+        self.nufft_type = int(nufft_type)
+        self.dtype = dtype
+        self.device = torch.device(device or "cuda")
+        self.real_dtype = torch.float32 if dtype == torch.complex64 else torch.float64
+        self.n_modes = tuple(int(n) for n in n_modes) if nufft_type != 3 else ()
+        self.ndim = len(self.n_modes) if nufft_type != 3 else int(native_opts.pop("ndim"))
+        self.ntransf = int(ntransf)
+        o = JfaOpts()
+        L.jfa_default_opts(ctypes.byref(o))
+        for k, v in native_opts.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown option {k}")
+            setattr(o, k, v)
+        o.gpu_stream = torch.cuda.current_stream(self.device).cuda_stream
+        nm = (c_int64 * 3)(*([*self.n_modes[::-1]] + [1] * (3 - len(self.n_modes))))
+        with torch.cuda.device(self.device):
+            rc = L.jfa_makeplan(self.nufft_type, self.ndim, nm, int(iflag), self.ntransf,
+                                float(eps), int(dtype == torch.complex128), ctypes.byref(o),
+                                ctypes.byref(self._plan))
+        self.warn = check(rc, "makeplan")
+        self._keep = ()
+
+    def info(self):
+        ns = c_int()
+        beta = c_double()
+        nf = (c_int64 * 3)()
+        lib().jfa_plan_info(self._plan, ctypes.byref(ns), ctypes.byref(beta), nf)
+        return {"ns": ns.value, "beta": beta.value, "nf": tuple(nf[: self.ndim])[::-1]}
+
+    def _sync_stream(self):
+        lib().jfa_set_stream(self._plan, torch.cuda.current_stream(self.device).cuda_stream)
+
+    def setpts(self, *points, targets=()):
+        pts = [p.contiguous() for p in points][::-1]
+        tg = [t.contiguous() for t in targets][::-1]
+        for p in pts + tg:
+            if p.dtype != self.real_dtype:
+                raise TypeError(f"points must be {self.real_dtype}")
+        _require_device(*(pts + tg))
+        M = pts[0].numel()
+        N = tg[0].numel() if tg else 0
+        ptrs = [c_void_p(p.data_ptr()) for p in pts] + [c_void_p(None)] * (3 - len(pts))
+        tptrs = [c_void_p(t.data_ptr()) for t in tg] + [c_void_p(None)] * (3 - len(tg))
+        self._keep = (pts, tg)  # the backend reads type-3 coordinates at execute time
+        with torch.cuda.device(self.device):
+            self._sync_stream()
+            check(lib().jfa_setpts(self._plan, M, *ptrs, N, *tptrs), "setpts")
+        self.M, self.N = M, N
+
+    def execute(self, c, fk):
+        with torch.cuda.device(self.device):
+            self._sync_stream()
+            check(lib().jfa_execute(self._plan, c_void_p(c.data_ptr()),
+                                    c_void_p(fk.data_ptr())), "execute")
+
+    def destroy(self):
+        if self._plan and self._plan.value:
+            lib().jfa_destroy(self._plan)
+            self._plan = c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,311 @@
+"""Public transforms ``nufft1``, ``nufft2``, ``nufft3``.
+
+Same signatures, defaults, validation and broadcasting as
+src/jax_finufft/ops.py:50-155 (``nufft1(output_shape, source, *points,
+iflag=1, eps=1e-6, opts=None)``, ``nufft2(source, *points, iflag=-1, ...)``,
+``nufft3(source, *points, iflag=-1, ...)``), on torch tensors living on a HIP
+device.  NumPy inputs are moved to the current HIP device.
+
+Differentiation (torch autograd, reverse mode) follows the reference's rules
+(jvp ops.py:158-277 and transpose :280-314), translated to torch's complex
+gradient convention (the gradient of a real loss w.r.t. a complex z is
+dL/dRe z + i dL/dIm z, i.e. the conjugate of JAX's cotangent):
+
+* type 1  f = A c:  grad_c = A^H G = nufft2(G, x, iflag=-iflag);
+  grad_x_d = sum_t Re(c * conj(d/dx_d [nufft2(G, ., iflag=-iflag)](x)))
+* type 2  c = B f:  grad_f = nufft1(g, x, iflag=-iflag);
+  grad_x_d = sum_t Re(conj(g) * d/dx_d [nufft2(f, ., iflag)](x))
+* type 3: the reference's stacked type-3 rule with sources and targets swapped.
+
+With ``Opts(gpu_fused_adjoint=True)`` (the default) the point derivatives
+d/dx_d of a type-2 evaluation come from one fused interpolation pass that
+uses the analytic kernel derivative (all d at once, from the same fine grid);
+with ``False`` they come from the reference's stacked transforms weighted by
+i*iflag*k_d.  ``torch.func.vmap`` uses the reference's batching rule
+(ops.py:317-353).
+"""
+
+__all__ = ["nufft1", "nufft2", "nufft3", "get_frequency_array"]
+
+import numpy as np
+import torch
+
+from . import _native, options, shapes
+
+
+def get_frequency_array(n, modeord):
+    """Frequencies of a mode axis of length n (src/jax_finufft/ops.py:114-123)."""
+    if modeord == 0:
+        return np.arange(-(n // 2), (n + 1) // 2)
+    elif modeord == 1:
+        f = np.empty(n, dtype=np.int64)
+        f[: (n + 1) // 2] = np.arange(0, (n + 1) // 2)
+        f[(n + 1) // 2:] = np.arange(-(n // 2), 0)
+        return f
+    else:
+        raise ValueError(f"Unsupported modeord: {modeord}")
+
+
+def _default_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("jax_finufft_amd needs a HIP device; none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_tensor(a, device):
+    if isinstance(a, torch.Tensor):
+        return a
+    arr = np.asarray(a)
+    return torch.as_tensor(arr, device=device)
+
+
+def _inputs(source, points):
+    dev = None
+    for a in (source, *points):
+        if isinstance(a, torch.Tensor):
+            dev = a.device
+            break
+    if dev is None:
+        dev = _default_device()
+    return _as_tensor(source, dev), [_as_tensor(p, dev) for p in points]
+
+
+def _modeord(opts):
+    return int(opts.modeord) if hasattr(opts, "modeord") else 0
+
+
+# ------------------------------------------------------------------ primitive
+
+
+class _Primitive(torch.autograd.Function):
+    """One canonical-layout transform through the C-ABI (the ``nufft*_p``
+    primitives of ops.py:356-386)."""
+
+    @staticmethod
+    def forward(nufft_type, output_shape, iflag, eps, opts, source, *points):
+        return _native.lowering(source, *points, output_shape=output_shape, iflag=iflag,
+                                eps=eps, opts=opts, nufft_type=nufft_type)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        nufft_type, output_shape, iflag, eps, opts, source, *points = inputs
+        ctx.nufft_type = nufft_type
+        ctx.output_shape = output_shape
+        ctx.iflag = iflag
+        ctx.eps = eps
+        ctx.opts = opts
+        ctx.save_for_backward(source, *points)
+
+    @staticmethod
+    def backward(ctx, grad):
+        source, *points = ctx.saved_tensors
+        need = ctx.needs_input_grad[5:]
+        grads = _adjoint(ctx.nufft_type, grad, source, points, ctx.output_shape, ctx.iflag,
+                         ctx.eps, ctx.opts, need)
+        return (None, None, None, None, None, *grads)
+
+    @staticmethod
+    def vmap(info, in_dims, nufft_type, output_shape, iflag, eps, opts, source, *points):
+        return _batch(in_dims[5:], nufft_type, output_shape, iflag, eps, opts, source, points)
+
+
+def _bind(nufft_type, source, points, output_shape, iflag, eps, opts):
+    return _Primitive.apply(nufft_type, output_shape, iflag, eps, opts, source, *points)
+
+
+# ------------------------------------------------------------------ public API
+
+
+def nufft1(output_shape, source, *points, iflag=1, eps=1e-6, opts=None):
+    iflag = int(iflag)
+    eps = float(eps)
+    ndim = len(points)
+    if not 1 <= ndim <= 3:
+        raise ValueError("Only 1-, 2-, and 3-dimensions are supported")
+    output_shape = np.atleast_1d(output_shape).astype(np.int64)
+    if output_shape.shape != (ndim,):
+        raise ValueError(f"output_shape must have shape: ({ndim},)")
+    output_shape = tuple(int(n) for n in output_shape)
+    source, points = _inputs(source, points)
+    index, source, *points = shapes.broadcast_and_flatten_inputs(1, output_shape, source, *points)
+    result = _bind(1, source, points, output_shape, iflag, eps, opts)
+    return index.unflatten(result)
+
+
+def nufft2(source, *points, iflag=-1, eps=1e-6, opts=None):
+    iflag = int(iflag)
+    eps = float(eps)
+    ndim = len(points)
+    if not 1 <= ndim <= 3:
+        raise ValueError("Only 1-, 2-, and 3-dimensions are supported")
+    source, points = _inputs(source, points)
+    index, source, *points = shapes.broadcast_and_flatten_inputs(2, None, source, *points)
+    result = _bind(2, source, points, None, iflag, eps, opts)
+    return index.unflatten(result)
+
+
+def nufft3(source, *points, iflag=-1, eps=1e-6, opts=None):
+    iflag = int(iflag)
+    eps = float(eps)
+    twice_ndim = len(points)
+    if twice_ndim % 2 != 0:
+        raise ValueError("nufft3 requires an even number of point arrays")
+    ndim = twice_ndim // 2
+    if not 1 <= ndim <= 3:
+        raise ValueError("Only 1-, 2-, and 3-dimensions are supported")
+    source, points = _inputs(source, points)
+    index, source, *points = shapes.broadcast_and_flatten_inputs(3, None, source, *points)
+    result = _bind(3, source, points, None, iflag, eps, opts)
+    return index.unflatten(result)
+
+
+# ------------------------------------------------------------------ adjoints
+
+
+def _freq_weights(n_modes, dim, modeord, scale, like):
+    """i*scale*k_d broadcast over a (…, *n_modes) mode array."""
+    ndim = len(n_modes)
+    k = torch.as_tensor(get_frequency_array(n_modes[dim], modeord), device=like.device)
+    shape = [1] * ndim
+    shape[dim] = -1
+    return (1j * scale * k.to(like.real.dtype)).reshape(shape).to(like.dtype)
+
+
+def _stack(arrs):
+    # (n_tot, n_tr, ...) x K -> (n_tot, K*n_tr, ...), transform-major blocks
+    return torch.cat(arrs, dim=1)
+
+
+def _unstack(a, k):
+    return torch.chunk(a, k, dim=1)
+
+
+def _adjoint(nufft_type, grad, source, points, output_shape, iflag, eps, opts, need):
+    """Gradients w.r.t. (source, *points) of a canonical-layout transform."""
+    grad = grad.contiguous()
+    if nufft_type == 3:
+        return _adjoint3(grad, source, points, iflag, eps, opts, need)
+    ndim = len(points)
+    need_src, need_pts = need[0], need[1:]
+    want_pts = any(need_pts)
+    fopts = options.unpack_opts(opts, nufft_type, True)
+    fused = want_pts and bool(getattr(fopts, "gpu_fused_adjoint", True)) and _fused_available()
+    modeord = _modeord(fopts)
+    out = [None] * (1 + ndim)
+
+    if nufft_type == 1:
+        # grad_c = nufft2(G, x, -iflag); point grads from d/dx of the same evaluation
+        bopts = options.unpack_opts(opts, 2, False)
+        n_modes = tuple(output_shape)
+        if want_pts and fused:
+            val, dval = _fused_interp_grad(grad, points, -iflag, eps, bopts)
+            if need_src:
+                out[0] = val
+            for d in range(ndim):
+                if need_pts[d]:
+                    # dval[d] is d/dx of nufft2(G, ., -iflag) at the points
+                    out[1 + d] = (source * dval[d].conj()).real.sum(dim=1)
+            return out
+        args = [grad]
+        if want_pts:
+            for d in range(ndim):
+                args.append(grad * _freq_weights(n_modes, d, modeord, -iflag, grad))
+        res = _bind(2, _stack(args).contiguous(), points, None, -iflag, eps, bopts)
+        parts = _unstack(res, len(args))
+        if need_src:
+            out[0] = parts[0]
+        for d in range(ndim):
+            if want_pts and need_pts[d]:
+                out[1 + d] = (source * parts[1 + d].conj()).real.sum(dim=1)
+        return out
+
+    # type 2
+    n_modes = tuple(source.shape[2:])
+    if need_src:
+        bopts = options.unpack_opts(opts, 1, False)
+        out[0] = _bind(1, grad, points, n_modes, -iflag, eps, bopts)
+    if want_pts:
+        if fused:
+            _, dval = _fused_interp_grad(source, points, iflag, eps, fopts)
+            derivs = dval
+        else:
+            args = [source * _freq_weights(n_modes, d, modeord, iflag, source)
+                    for d in range(ndim)]
+            res = _bind(2, _stack(args).contiguous(), points, None, iflag, eps, fopts)
+            derivs = _unstack(res, ndim)
+        for d in range(ndim):
+            if need_pts[d]:
+                out[1 + d] = (grad.conj() * derivs[d]).real.sum(dim=1)
+    return out
+
+
+def _adjoint3(grad, source, points, iflag, eps, opts, need):
+    ndim = len(points) // 2
+    x, s = list(points[:ndim]), list(points[ndim:])
+    need_src, need_x, need_s = need[0], need[1:1 + ndim], need[1 + ndim:]
+    out = [None] * (1 + 2 * ndim)
+    bopts = options.unpack_opts(opts, 3, False)
+    # grad_c and grad_x: sources s with strengths [G, -i iflag s_d G], targets x
+    args = [grad] if need_src else []
+    want_x = any(need_x)
+    if want_x:
+        for d in range(ndim):
+            args.append(grad * (-1j * iflag) * s[d][:, None, :].to(grad.dtype))
+    if args:
+        res = _bind(3, _stack(args).contiguous(), s + x, None, -iflag, eps, bopts)
+        parts = list(_unstack(res, len(args)))
+        if need_src:
+            out[0] = parts.pop(0)
+        for d in range(ndim):
+            if want_x and need_x[d]:
+                out[1 + d] = (source * parts[d].conj()).real.sum(dim=1)
+    # grad_s: sources x with strengths [i iflag x_d c], targets s
+    if any(need_s):
+        fopts = options.unpack_opts(opts, 3, True)
+        args = [source * (1j * iflag) * x[d][:, None, :].to(source.dtype) for d in range(ndim)]
+        res = _bind(3, _stack(args).contiguous(), x + s, None, iflag, eps, fopts)
+        parts = _unstack(res, ndim)
+        for d in range(ndim):
+            if need_s[d]:
+                out[1 + ndim + d] = (grad.conj() * parts[d]).real.sum(dim=1)
+    return out
+
+
+def _fused_available():
+    return hasattr(_native, "interp_grad")
+
+
+def _fused_interp_grad(modes, points, iflag, eps, opts):
+    return _native.interp_grad(modes, points, iflag=iflag, eps=eps, opts=opts)
+
+
+# ------------------------------------------------------------------ vmap
+
+
+def _batch(in_dims, nufft_type, output_shape, iflag, eps, opts, source, points):
+    """The reference's batching rule (ops.py:317-353)."""
+    bsource, *bpoints = in_dims
+    kwargs = dict(iflag=iflag, eps=eps, opts=opts)
+    if all(bx is None for bx in bpoints):
+        assert bsource is not None
+        source = torch.movedim(source, bsource, 0)
+        mapped_points = tuple(p[None] for p in points)
+    else:
+        if bsource is None:
+            num_repeats = next(x.shape[bx] for x, bx in zip(points, bpoints) if bx is not None)
+            source = source[None].expand((num_repeats,) + tuple(source.shape))
+        else:
+            num_repeats = source.shape[bsource]
+            source = torch.movedim(source, bsource, 0)
+        mapped_points = []
+        for x, bx in zip(points, bpoints):
+            if bx is None:
+                mapped_points.append(x[None].expand((num_repeats,) + tuple(x.shape)))
+            else:
+                mapped_points.append(torch.movedim(x, bx, 0))
+    # the canonical primitive's own leading axes ride along as extra batch axes
+    if nufft_type == 3:
+        return nufft3(source, *mapped_points, **kwargs), 0
+    elif nufft_type == 2:
+        return nufft2(source, *mapped_points, **kwargs), 0
+    return nufft1(tuple(output_shape), source, *mapped_points, **kwargs), 0

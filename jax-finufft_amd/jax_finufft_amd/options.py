@@ -1,0 +1,159 @@
+"""Transform options, mirroring the reference's option dataclasses.
+
+Same fields, defaults and resolution rules as src/jax_finufft/options.py:46-148
+(``Opts``, ``NestedOpts``, ``unpack_opts``).  Differences:
+
+* ``to_cufinufft_opts`` (options.py:105-119) returns the seven GPU fields the
+  reference forwards to its handlers; this backend's handlers take exactly the
+  same seven attributes.  ``to_native_opts`` additionally carries the GPU
+  fields the reference declares but drops (bin sizes, ``gpu_maxsubprobsize``,
+  ``gpu_spreadinterponly``), which the plan-level API here does honour.
+* ``gpu_fused_adjoint`` (new): reverse mode uses the fused adjoint kernels
+  (one transform plus one gradient-interpolation pass) instead of the
+  reference's stacked multi-NUFFT rule (ops.py:158-314).
+* FFTW flag values are FFTW's public constants (the reference reads them from
+  its CPU extension, options.py:20-25).
+"""
+
+from enum import IntEnum
+from typing import Optional, Union
+
+from pydantic.dataclasses import dataclass
+
+
+class DebugLevel(IntEnum):
+    Silent = 0
+    Verbose = 1
+    Noisy = 2
+
+
+class GpuDebugLevel(IntEnum):
+    Silent = 0
+    Verbose = 1
+
+
+class FftwFlags(IntEnum):
+    Estimate = 64
+    Measure = 0
+    Patient = 32
+    Exhaustive = 8
+    WisdomOnly = 2097152
+
+
+class SpreadSort(IntEnum):
+    NoSort = 0
+    Sort = 1
+    Heuristic = 2
+
+
+class SpreadThread(IntEnum):
+    Auto = 0
+    Sequential = 1
+    Parallel = 2
+
+
+class GpuMethod(IntEnum):
+    Auto = 0
+    Driven = 1  # global-memory atomics in sorted order
+    Shared = 2  # LDS subproblem tiles
+
+
+@dataclass(frozen=True)
+class Opts:
+    # CPU fields (kept for interface parity; the CPU path is not part of this backend)
+    modeord: bool = False
+    debug: DebugLevel = DebugLevel.Silent
+    spread_debug: DebugLevel = DebugLevel.Silent
+    showwarn: bool = False
+    nthreads: int = 0
+    fftw: int = FftwFlags.Estimate
+    spread_sort: SpreadSort = SpreadSort.Heuristic
+    spread_kerevalmeth: bool = True
+    spread_kerpad: bool = True
+    upsampfac: float = 0.0
+    spread_thread: SpreadThread = SpreadThread.Auto
+    maxbatchsize: int = 0
+    spread_nthr_atomic: int = -1
+    spread_max_sp_size: int = 0
+
+    # GPU fields
+    gpu_upsampfac: float = 2.0
+    gpu_method: GpuMethod = 0
+    gpu_sort: bool = True
+    gpu_binsizex: int = 0
+    gpu_binsizey: int = 0
+    gpu_binsizez: int = 0
+    gpu_obinsizex: int = 0
+    gpu_obinsizey: int = 0
+    gpu_obinsizez: int = 0
+    gpu_maxsubprobsize: int = 1024
+    gpu_kerevalmeth: bool = True
+    gpu_spreadinterponly: bool = False
+    gpu_maxbatchsize: int = 0
+    gpu_debug: GpuDebugLevel = GpuDebugLevel.Silent
+
+    # additions of this backend
+    gpu_fused_adjoint: bool = True
+
+    def to_cufinufft_opts(self):
+        """The seven attributes the handlers take (same set as the reference)."""
+
+        class NativeOpts:
+            pass
+
+        opts = NativeOpts()
+        opts.modeord = int(self.modeord)
+        opts.upsampfac = float(self.gpu_upsampfac)
+        opts.gpu_method = int(self.gpu_method)
+        opts.gpu_sort = int(self.gpu_sort)
+        opts.gpu_kerevalmeth = int(self.gpu_kerevalmeth)
+        opts.gpu_maxbatchsize = int(self.gpu_maxbatchsize)
+        opts.debug = int(self.gpu_debug)
+        return opts
+
+    def to_native_opts(self):
+        """All plan-level GPU options, as a dict of ``jfa_opts`` field values."""
+        return {
+            "modeord": int(self.modeord),
+            "upsampfac": float(self.gpu_upsampfac),
+            "gpu_method": int(self.gpu_method),
+            "gpu_sort": int(self.gpu_sort),
+            "gpu_kerevalmeth": int(self.gpu_kerevalmeth),
+            "gpu_maxbatchsize": int(self.gpu_maxbatchsize),
+            "debug": int(self.gpu_debug),
+            "gpu_binsizex": int(self.gpu_binsizex),
+            "gpu_binsizey": int(self.gpu_binsizey),
+            "gpu_binsizez": int(self.gpu_binsizez),
+            "gpu_maxsubprobsize": int(self.gpu_maxsubprobsize),
+            "gpu_spreadinterponly": int(self.gpu_spreadinterponly),
+        }
+
+
+@dataclass(frozen=True)
+class NestedOpts:
+    type1: Optional[Opts] = None
+    type2: Optional[Opts] = None
+    type3: Optional[Opts] = None
+
+    forward: Optional[Opts] = None
+    backward: Optional[Union[Opts, "NestedOpts"]] = None
+
+
+def unpack_opts(opts, finufft_type, forward):
+    """Pick the options for one transform (src/jax_finufft/options.py:132-148)."""
+    if opts is None or isinstance(opts, Opts):
+        return opts
+
+    if forward:
+        if opts.forward is not None:
+            return opts.forward
+        elif finufft_type == 1:
+            return opts.type1
+        elif finufft_type == 2:
+            return opts.type2
+        elif finufft_type == 3:
+            return opts.type3
+    elif opts.backward is not None:
+        return opts.backward
+
+    return opts

@@ -1,0 +1,118 @@
+"""The C-ABI library: it loads, exports every symbol include/*.h declares, and
+its host-only plan-time math agrees with the oracle's restatement and with the
+mathematics (no device work in this file)."""
+
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import finufft_cpu as fc
+from jax_finufft_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
+        names.update(re.findall(r"\b(jfa_\w+)\s*\(", text))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def L():
+    return _native.lib()
+
+
+def test_header_declares_plan_api_and_18_entry_points():
+    names = declared_symbols()
+    for n in _native.ENTRY_POINTS + _native.PLAN_API:
+        assert n in names
+    assert len([n for n in names if re.fullmatch(r"jfa_nufft[123]d[123]f?", n)]) == 18
+
+
+def test_library_exports_every_declared_symbol(L):
+    missing = [n for n in declared_symbols() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _params(L, eps, sigma=2.0, rule=0, dbl=1):
+    ns, beta, nc = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+    rc = L.jfa_kernel_params(ctypes.c_double(eps), ctypes.c_double(sigma), rule, dbl,
+                             ctypes.byref(ns), ctypes.byref(beta), ctypes.byref(nc))
+    return rc, ns.value, beta.value, nc.value
+
+
+@pytest.mark.parametrize("eps", [1e-2, 1e-3, 3e-5, 1e-6, 1e-9, 1e-12, 1e-14])
+@pytest.mark.parametrize("sigma", [2.0, 1.25])
+def test_width_rule_1_equals_published_rule(L, eps, sigma):
+    rc, ns, beta, _ = _params(L, eps, sigma, rule=1)
+    ons, obeta, orc = fc.params(eps, sigma)
+    assert (ns, rc) == (ons, orc)
+    assert beta == pytest.approx(obeta, rel=1e-15)
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-6, 2e-6, 1e-9])
+def test_default_width_rule_is_published_rule_at_half_eps(L, eps):
+    _, ns, beta, _ = _params(L, eps, rule=0)
+    assert ns == fc.params(eps / 2)[0]
+
+
+def test_single_precision_eps_clamp(L):
+    rc, ns, _, _ = _params(L, 1e-9, dbl=0)
+    assert rc == 1  # the eps-too-small warning
+    assert ns == _params(L, np.finfo(np.float32).eps, dbl=0)[1]
+
+
+@pytest.mark.parametrize("n,ns", [(1, 2), (7, 4), (75, 8), (1000, 7), (1024, 7), (2 ** 16, 10)])
+def test_fine_grid_size_matches_oracle(L, n, ns):
+    L.jfa_fine_grid_size.restype = ctypes.c_int64
+    L.jfa_fine_grid_size.argtypes = [ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+    assert L.jfa_fine_grid_size(n, 2.0, ns) == fc.fine_size(n, ns)
+    assert L.jfa_fine_grid_size(n, 1.25, ns) == fc.fine_size(n, ns, 1.25)
+
+
+@pytest.mark.parametrize("eps", [1e-2, 1e-4, 1e-6, 1e-9, 1e-12])
+def test_horner_table_reproduces_kernel(L, eps):
+    _, ns, beta, nc = _params(L, eps)
+    coef = np.zeros(nc * ns)
+    assert L.jfa_horner_table(ns, nc, ctypes.c_double(beta),
+                              coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    coef = coef.reshape(nc, ns)
+    t = np.linspace(-1, 1, 1001)
+    worst = 0.0
+    for i in range(ns):
+        v = np.polynomial.polynomial.polyval(t, coef[:, i])
+        r = -ns / 2 + i + (t + 1) / 2
+        z = 2 * r / ns
+        exact = np.exp(beta * (np.sqrt(np.clip(1 - z * z, 0, None)) - 1))
+        worst = max(worst, np.max(np.abs(v - exact)))
+    # the fit error stays below the kernel's own truncation value exp(-beta)
+    assert worst < max(np.exp(-beta), 1e-14)
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-6, 1e-10])
+def test_kernel_fourier_matches_oracle(L, eps):
+    _, ns, beta, _ = _params(L, eps)
+    nf, n = 96, 47
+    out = np.zeros(n)
+    L.jfa_kernel_fourier.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int64,
+                                     ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.POINTER(ctypes.c_double)]
+    assert L.jfa_kernel_fourier(ns, beta, nf, -(n // 2), n,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    np.testing.assert_allclose(out, fc.phihat(ns, beta, nf, n), rtol=1e-13)
+
+
+def test_bad_arguments_are_errors_not_crashes(L):
+    rc, *_ = _params(L, -1.0)
+    assert rc == 2  # JFA_ERR_BAD_ARG
+    assert b"eps" in L.jfa_last_error()

@@ -1,0 +1,234 @@
+"""GPU parity of the forward transforms (through the C-ABI entry points).
+
+* the reference's own forward tests (tests/ops_test.py:25-125), on the golden
+  fixtures re-created from its seeded constructions, with its tolerances;
+* the accuracy bar: relative l2 error against the FP64 direct sum <= eps, for
+  uniform and clustered points, every type/dim/precision;
+* modeord (ops_test.py:494-567) and multi-transform (:380-397) fixtures;
+* stacked / batched layouts against per-slice calls; edge cases.
+"""
+
+import os
+from itertools import product
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gpu_helpers import (CDT, NPC, NPR, RDT, clustered_points, cnormal, dev, host, rel_l2,
+                         uniform_points)
+from jax_finufft_amd import nufft1, nufft2, nufft3
+from jax_finufft_amd.options import Opts
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FWD = np.load(os.path.join(GOLD, "forward.npz"))
+MO = np.load(os.path.join(GOLD, "modeord.npz"))
+# reference tolerances (ops_test.py:14-16, :120), applied to the relative l2 error
+TOL = {"f32": 1e-4, "f64": 1e-7}
+TOL3 = {"f32": 1e-3, "f64": 1e-7}
+
+
+# ------------------------------------------------------- reference forward tests
+
+
+@pytest.mark.parametrize("ndim,prec,iflag", product([1, 2, 3], ["f32", "f64"], [-1, 1]))
+def test_nufft1_forward_golden(ndim, prec, iflag):
+    key = f"t1_d{ndim}_{prec}_i{iflag:+d}"
+    x, c, f = FWD[key + "_x"], FWD[key + "_c"], FWD[key + "_f"]
+    eps = 1e-10 if prec == "f64" else 1e-7
+    got = nufft1(f.shape, dev(c), *[dev(xx) for xx in x], eps=eps, iflag=iflag)
+    assert got.dtype == CDT[prec] and tuple(got.shape) == f.shape
+    assert rel_l2(host(got), f) < TOL[prec]
+
+
+@pytest.mark.parametrize("ndim,prec,iflag", product([1, 2, 3], ["f32", "f64"], [-1, 1]))
+def test_nufft2_forward_golden(ndim, prec, iflag):
+    key = f"t2_d{ndim}_{prec}_i{iflag:+d}"
+    x, f, c = FWD[key + "_x"], FWD[key + "_f"], FWD[key + "_c"]
+    eps = 1e-10 if prec == "f64" else 1e-7
+    got = nufft2(dev(f), *[dev(xx) for xx in x], eps=eps, iflag=iflag)
+    assert tuple(got.shape) == c.shape
+    assert rel_l2(host(got), c) < TOL[prec]
+
+
+@pytest.mark.parametrize("ndim,prec,iflag", product([1, 2, 3], ["f32", "f64"], [-1, 1]))
+def test_nufft3_forward_golden(ndim, prec, iflag):
+    key = f"t3_d{ndim}_{prec}_i{iflag:+d}"
+    x, s, c, f = FWD[key + "_x"], FWD[key + "_s"], FWD[key + "_c"], FWD[key + "_f"]
+    eps = 1e-10 if prec == "f64" else 1e-7
+    got = nufft3(dev(c), *[dev(a) for a in x], *[dev(a) for a in s], eps=eps, iflag=iflag)
+    assert tuple(got.shape) == f.shape
+    assert rel_l2(host(got), f) < TOL3[prec]
+
+
+@pytest.mark.parametrize("modeord,n,ndim,t", product([0, 1], [16, 17], [1, 2, 3], [1, 2]))
+def test_modeord_golden(modeord, n, ndim, t):
+    key = f"mo{modeord}_n{n}_d{ndim}_t{t}"
+    x = [dev(a) for a in MO[key + "_x"]]
+    opts = Opts(modeord=modeord)
+    if t == 1:
+        f = MO[key + "_f"]
+        got = nufft1(f.shape, dev(MO[key + "_c"]), *x, eps=1e-10, iflag=1, opts=opts)
+        assert rel_l2(host(got), f) < 1e-7
+    else:
+        got = nufft2(dev(MO[key + "_f"]), *x, eps=1e-10, iflag=1, opts=opts)
+        assert rel_l2(host(got), MO[key + "_c"]) < 1e-7
+
+
+def test_multi_transform_golden():
+    d = np.load(os.path.join(GOLD, "multi_transform.npz"))
+    c, f, x, xt = dev(d["mt_c"]), dev(d["mt_f"]), dev(d["mt_x"]), dev(d["mt_xt"])
+    # jax's default (non-x64) precision is what the reference test runs at
+    c32, f32, x32, xt32 = c.to(torch.complex64), f.to(torch.complex64), x.float(), xt.float()
+    assert rel_l2(host(nufft1(12, c32, x32)), d["mt_t1"]) < 1e-4
+    assert rel_l2(host(nufft2(f32, x32)), d["mt_t2"]) < 1e-4
+    assert rel_l2(host(nufft3(c32, x32, xt32)), d["mt_t3"]) < 1e-3
+    for n in range(c.shape[1]):
+        assert rel_l2(host(nufft1(12, c[:, n], x)), d["mt_t1"][:, n]) < 1e-7
+        assert rel_l2(host(nufft2(f[:, n], x)), d["mt_t2"][:, n]) < 1e-7
+        assert rel_l2(host(nufft3(c[:, n], x, xt)), d["mt_t3"][:, n]) < 1e-7
+
+
+# ------------------------------------------------------- accuracy bar vs direct sum
+
+MODES = {1: (301,), 2: (40, 52), 3: (14, 16, 18)}
+EPS = {"f32": [1e-3, 1e-5, 1e-6], "f64": [1e-6, 1e-9, 1e-12]}
+
+
+def _cases():
+    for ndim, prec, dist in product([1, 2, 3], ["f32", "f64"], ["uniform", "clustered"]):
+        for eps in EPS[prec]:
+            yield ndim, prec, dist, eps
+
+
+@pytest.mark.parametrize("ndim,prec,dist,eps", list(_cases()))
+@pytest.mark.parametrize("method", [1, 2])
+def test_eps_bar_type1_type2(ndim, prec, dist, eps, method):
+    rng = np.random.default_rng(100 + ndim)
+    M = 3000
+    gen = uniform_points if dist == "uniform" else clustered_points
+    x = gen(rng, ndim, M, prec)
+    c = cnormal(rng, M, prec)
+    n_modes = MODES[ndim]
+    f = cnormal(rng, n_modes, prec)
+    opts = Opts(gpu_method=method)
+    for iflag in (-1, 1):
+        got1 = host(nufft1(n_modes, dev(c), *[dev(a) for a in x], eps=eps, iflag=iflag,
+                           opts=opts))
+        ref1 = oracle.type1(x, c, n_modes, iflag)
+        got2 = host(nufft2(dev(f), *[dev(a) for a in x], eps=eps, iflag=iflag, opts=opts))
+        ref2 = oracle.type2(f, x, iflag)
+        assert rel_l2(got1, ref1) <= eps, (iflag, rel_l2(got1, ref1))
+        assert rel_l2(got2, ref2) <= eps, (iflag, rel_l2(got2, ref2))
+
+
+@pytest.mark.parametrize("ndim,prec", product([1, 2, 3], ["f32", "f64"]))
+@pytest.mark.parametrize("extent", [1.0, 30.0])
+def test_eps_bar_type3(ndim, prec, extent):
+    rng = np.random.default_rng(7 + ndim)
+    M, N = 1500, 1300
+    x = [rng.uniform(-extent, extent + 0.5, M).astype(NPR[prec]) for _ in range(ndim)]
+    s = [rng.uniform(-extent - 1.0, extent, N).astype(NPR[prec]) for _ in range(ndim)]
+    c = cnormal(rng, M, prec)
+    for eps in EPS[prec]:
+        got = host(nufft3(dev(c), *[dev(a) for a in x], *[dev(a) for a in s], eps=eps))
+        ref = oracle.type3(c, x, s, -1)
+        assert rel_l2(got, ref) <= eps, (eps, rel_l2(got, ref))
+
+
+# ------------------------------------------------------- layouts and edge cases
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_stacked_and_batched_layouts(ndim):
+    rng = np.random.default_rng(3)
+    M, T, B = 400, 3, 2
+    n_modes = MODES[ndim]
+    x = [rng.uniform(-np.pi, np.pi, (B, 1, M)) for _ in range(ndim)]
+    c = cnormal(rng, (B, T, M), "f64")
+    f = cnormal(rng, (B, T) + n_modes, "f64")
+    got1 = nufft1(n_modes, dev(c), *[dev(a) for a in x], eps=1e-9)
+    got2 = nufft2(dev(f), *[dev(a) for a in x], eps=1e-9)
+    assert tuple(got1.shape) == (B, T) + n_modes
+    assert tuple(got2.shape) == (B, T, M)
+    for b, t in product(range(B), range(T)):
+        xb = [dev(a[b, 0]) for a in x]
+        s1 = nufft1(n_modes, dev(c[b, t]), *xb, eps=1e-9)
+        s2 = nufft2(dev(f[b, t]), *xb, eps=1e-9)
+        assert rel_l2(host(got1[b, t]), host(s1)) < 1e-13
+        assert rel_l2(host(got2[b, t]), host(s2)) < 1e-13
+        ref = oracle.type1([a[b, 0] for a in x], c[b, t], n_modes, 1)
+        assert rel_l2(host(got1[b, t]), ref) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_folding_of_points_outside_the_period(prec):
+    rng = np.random.default_rng(4)
+    M = 500
+    x = [rng.uniform(-np.pi, np.pi, M) for _ in range(2)]
+    shift = [a + 2 * np.pi * rng.integers(-3, 4, M) for a in x]
+    c = cnormal(rng, M, prec)
+    a = host(nufft1((33, 20), dev(c), *[dev(v.astype(NPR[prec])) for v in x], eps=1e-6))
+    b = host(nufft1((33, 20), dev(c), *[dev(v.astype(NPR[prec])) for v in shift], eps=1e-6))
+    ref = oracle.type1([v.astype(NPR[prec]) for v in shift], c, (33, 20), 1)
+    assert rel_l2(b, ref) <= 1e-6
+    assert rel_l2(a, b) < (1e-6 if prec == "f32" else 1e-12)
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_all_points_in_one_cell(method):
+    # one crowded bin: many LDS subproblems over the same tile
+    M = 20000
+    x = np.full(M, 0.3)
+    y = np.full(M, -1.1)
+    rng = np.random.default_rng(5)
+    c = cnormal(rng, M, "f64")
+    got = host(nufft1((24, 31), dev(c), dev(x), dev(y), eps=1e-9, opts=Opts(gpu_method=method)))
+    ref = oracle.type1([x, y], c, (24, 31), 1)
+    assert rel_l2(got, ref) < 1e-9
+
+
+def test_zero_points():
+    x = torch.zeros(0, dtype=torch.float64, device="cuda")
+    c = torch.zeros(0, dtype=torch.complex128, device="cuda")
+    f = nufft1((7, 8), c, x, x)
+    assert tuple(f.shape) == (7, 8) and torch.count_nonzero(f) == 0
+    g = nufft2(torch.ones(7, 8, dtype=torch.complex128, device="cuda"), x, x)
+    assert tuple(g.shape) == (0,)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_tiny_mode_counts(n):
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-np.pi, np.pi, 50)
+    c = cnormal(rng, 50, "f64")
+    got = host(nufft1((n,), dev(c), dev(x), eps=1e-10))
+    assert rel_l2(got, oracle.type1([x], c, (n,), 1)) < 1e-9
+
+
+def test_single_point_closed_form():
+    # f_k = c e^{i k x} for one point: pins sign, index order and modeord at once
+    x0, y0 = 0.7, -2.2
+    c0 = 1.5 - 0.5j
+    n0, n1 = 9, 6
+    got = host(nufft1((n0, n1), dev(np.array([c0])), dev(np.array([x0])), dev(np.array([y0])),
+                      eps=1e-12, iflag=-1))
+    k0 = np.arange(-(n0 // 2), (n0 + 1) // 2)[:, None]
+    k1 = np.arange(-(n1 // 2), (n1 + 1) // 2)[None, :]
+    np.testing.assert_allclose(got, c0 * np.exp(-1j * (k0 * x0 + k1 * y0)), atol=1e-11)
+
+
+def test_bad_inputs_raise():
+    x = torch.zeros(5, dtype=torch.float32, device="cuda")
+    c = torch.zeros(5, dtype=torch.complex64, device="cuda")
+    with pytest.raises(ValueError):
+        nufft1((4, 4, 4, 4), c, x, x, x, x)
+    with pytest.raises(ValueError):
+        nufft1((4,), c, x, x)
+    with pytest.raises(TypeError):
+        nufft1((4,), c.to(torch.complex128), x)
+    with pytest.raises(ValueError):
+        nufft3(c, x, x, x)
