@@ -109,6 +109,15 @@ int jfa_destroy(jfa_plan plan);
 /* Kernel parameters the plan chose: width, beta, fine grid (x-fastest). */
 int jfa_plan_info(jfa_plan plan, int *ns, double *beta, int64_t *nf /* [3] */);
 
+/* Stage timing of a plan made with opts.debug == 2, which records HIP events
+ * around every stage on the plan's stream without synchronising.  Returns the
+ * accumulated milliseconds and launch count of `stage` ("setpts", "zero",
+ * "spread", "fft", "deconv", "amplify", "interp"); waits for the recorded
+ * events.  reset != 0 clears the accumulators afterwards.  (No reference
+ * counterpart: the reference's debug option only prints, lowering.py:139.) */
+int jfa_plan_timing(jfa_plan plan, const char *stage, int reset, double *total_ms,
+                    int64_t *count);
+
 const char *jfa_last_error(void);
 
 /* Number of plans held by the per-device plan cache, and a way to drop them. */

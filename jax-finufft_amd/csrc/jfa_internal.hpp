@@ -15,6 +15,9 @@ namespace jfa {
 
 constexpr int kMaxNs = 16;  // widest ES kernel supported (taps per dimension)
 constexpr int kMaxNc = 19;  // most Horner coefficients per kernel interval
+// Largest fine grid (cells per transform) a plan accepts: the FFT sizes this
+// backend validates.
+constexpr int64_t kMaxFineCells = (int64_t)1 << 32;
 
 // Horner coefficients per interval for width ns: degree ns+4, capped at 18.
 // (Chebyshev-interpolation error at this degree is ~exp(-beta)/2, i.e. below
@@ -54,6 +57,7 @@ const char *last_error();
         if (_e != hipSuccess) {                                                          \
             ::jfa::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),      \
                              __FILE__, __LINE__);                                        \
+            (void)hipGetLastError(); /* reported here; do not leak into later checks */  \
             return JFA_ERR_HIP;                                                          \
         }                                                                                \
     } while (0)

@@ -186,20 +186,36 @@ int jfa::fft_exec(jfa_plan p, void *data, int nbatch) {
 // ------------------------------------------------------------------ timing
 
 namespace jfa {
+// Per-stage timing on the plan's stream.  debug = 1 prints each call's stages
+// to stderr (synchronising); debug = 2 only records event pairs, which
+// jfa_plan_timing() resolves later, so timed loops are not perturbed.
 struct StageTimer {
     jfa_plan p;
     bool on;
     std::vector<std::pair<const char *, hipEvent_t>> marks;
     explicit StageTimer(jfa_plan pl) : p(pl), on(pl->opts.debug > 0) { mark("start"); }
+    ~StageTimer() {
+        for (auto &m : marks) (void)hipEventDestroy(m.second);
+    }
     void mark(const char *name) {
         if (!on) return;
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
         (void)hipEventRecord(e, p->stream);
         marks.push_back({name, e});
     }
     void report(const char *what) {
         if (!on || marks.size() < 2) return;
+        if (p->opts.debug >= 2) {
+            for (size_t i = 1; i < marks.size(); ++i)
+                p->timing_pairs.push_back({marks[i].first, marks[i - 1].second, marks[i].second});
+            for (auto &m : marks) p->timing_events.push_back(m.second);
+            marks.clear();
+            return;
+        }
         (void)hipEventSynchronize(marks.back().second);
         fprintf(stderr, "[jfa %s] type %d, %dd, ntransf=%d, M=%lld:", what, p->type, p->dim,
                 p->ntransf, (long long)p->M);
@@ -214,6 +230,40 @@ struct StageTimer {
     }
 };
 }  // namespace jfa
+
+extern "C" int jfa_plan_timing(jfa_plan p, const char *stage, int reset, double *total_ms,
+                               int64_t *count) {
+    if (!p) {
+        set_error("plan is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    if (!p->timing_pairs.empty()) {
+        JFA_HIP_CHECK(hipEventSynchronize(p->timing_events.back()));
+        for (auto &tp : p->timing_pairs) {
+            float ms = 0;
+            JFA_HIP_CHECK(hipEventElapsedTime(&ms, tp.a, tp.b));
+            auto &acc = p->timing_acc[tp.name];
+            acc.first += ms;
+            acc.second += 1;
+        }
+        for (hipEvent_t e : p->timing_events) (void)hipEventDestroy(e);
+        p->timing_pairs.clear();
+        p->timing_events.clear();
+    }
+    double t = 0;
+    int64_t n = 0;
+    if (stage) {
+        auto it = p->timing_acc.find(stage);
+        if (it != p->timing_acc.end()) {
+            t = it->second.first;
+            n = it->second.second;
+        }
+    }
+    if (total_ms) *total_ms = t;
+    if (count) *count = n;
+    if (reset) p->timing_acc.clear();
+    return JFA_OK;
+}
 
 // ------------------------------------------------------------------ plan API
 
@@ -350,6 +400,12 @@ extern "C" int jfa_makeplan(int type, int dim, const int64_t *nmodes, int iflag,
         }
         p->Ntot = p->nmodes[0] * p->nmodes[1] * p->nmodes[2];
         p->F = g.nf[0] * g.nf[1] * g.nf[2];
+        if (p->F > kMaxFineCells) {
+            jfa_destroy(p);
+            set_error("fine grid of %lld cells per transform exceeds the supported 2^32",
+                      (long long)(g.nf[0] * g.nf[1] * g.nf[2]));
+            return JFA_ERR_TOO_LARGE;
+        }
         default_bins(p);
         p->method = resolve_method(p);
         if (p->method < 0) {
@@ -364,6 +420,7 @@ extern "C" int jfa_makeplan(int type, int dim, const int64_t *nmodes, int iflag,
         if (!o.gpu_spreadinterponly) {
             const size_t bytes = (size_t)p->batch * p->F * 2 * (p->is_double ? 8 : 4);
             if (hipMalloc(&p->d_fw, bytes) != hipSuccess) {
+                (void)hipGetLastError();
                 jfa_destroy(p);
                 set_error("could not allocate the %zu-byte fine grid", bytes);
                 return JFA_ERR_ALLOC;
@@ -516,6 +573,7 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
         if (p->type == 1) {
             if ((rc = launch_fill_zero(fk, sizeof(T) * 2 * p->F * p->ntransf, p->stream)))
                 return rc;
+            tm.mark("zero");
             rc = launch_spread<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, c, M, fk, p->F,
                                   p->ntransf, p->stream);
             tm.mark("spread");
@@ -534,6 +592,7 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
         T *fb = fk + 2 * (int64_t)t0 * p->Ntot;
         if (p->type == 1) {
             if ((rc = launch_fill_zero(fw, sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
+            tm.mark("zero");
             if ((rc = launch_spread<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, cb, M, fw,
                                        p->F, nb, p->stream)))
                 return rc;
@@ -624,6 +683,7 @@ extern "C" int jfa_destroy(jfa_plan p) {
                      (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub})
         free_ptr(*q);
     jfa::type3_free(p);
+    for (hipEvent_t e : p->timing_events) (void)hipEventDestroy(e);
     if (p->order_event) (void)hipEventDestroy(p->order_event);
     delete p;
     return JFA_OK;

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <map>
+#include <string>
+#include <vector>
 
 #include "jfa_internal.hpp"
 
@@ -20,6 +22,15 @@ struct jfa_plan_s {
     int device = 0;
     bool used = false;                 // work has been queued on `stream`
     hipEvent_t order_event = nullptr;  // orders work across jfa_set_stream
+
+    // stage timing (opts.debug == 2): recorded event pairs, resolved on query
+    struct TimingPair {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<TimingPair> timing_pairs;
+    std::vector<hipEvent_t> timing_events;
+    std::map<std::string, std::pair<double, int64_t>> timing_acc;
 
     // kernel tables
     void *d_coef = nullptr;            // Horner coefficients (nc x ns)

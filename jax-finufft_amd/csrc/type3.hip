@@ -137,6 +137,18 @@ static int type3_setpts_t(jfa_plan p, int64_t M, const T *const xs[3], int64_t N
     }
     p->F = g.nf[0] * g.nf[1] * g.nf[2];
     p->Ntot = N;
+    {
+        // the inner type-2 plan upsamples the spread grid once more; check both
+        // before allocating anything
+        int64_t inner_cells = 1;
+        for (int d = 0; d < dim; ++d) inner_cells *= fine_size(g.nf[d], sigma, ns);
+        if (p->F > kMaxFineCells || inner_cells > kMaxFineCells) {
+            set_error("type-3 grids of %lld / %lld cells exceed the supported 2^32 "
+                      "(space-bandwidth product too large for one transform)",
+                      (long long)p->F, (long long)inner_cells);
+            return JFA_ERR_TOO_LARGE;
+        }
+    }
     // bins and method for the new grid
     {
         const int b2 = 32, b1 = 1024;
@@ -166,6 +178,7 @@ static int type3_setpts_t(jfa_plan p, int64_t M, const T *const xs[3], int64_t N
         free_ptr(p->d_fw);
         const size_t bytes = (size_t)p->F * p->batch * 2 * sizeof(T);
         if (hipMalloc(&p->d_fw, bytes) != hipSuccess) {
+            (void)hipGetLastError();
             set_error("could not allocate the %zu-byte type-3 grid", bytes);
             return JFA_ERR_ALLOC;
         }

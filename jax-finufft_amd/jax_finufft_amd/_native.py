@@ -98,6 +98,8 @@ def lib():
                                     ctypes.POINTER(c_int64)]
         L.jfa_cache_size.restype = c_int
         L.jfa_cache_clear.restype = None
+        L.jfa_plan_timing.argtypes = [c_void_p, ctypes.c_char_p, c_int,
+                                      ctypes.POINTER(c_double), ctypes.POINTER(c_int64)]
         for name in ENTRY_POINTS:
             fn = getattr(L, name)
             single = name.endswith("f")
@@ -241,6 +243,14 @@ class Plan:
         nf = (c_int64 * 3)()
         lib().jfa_plan_info(self._plan, ctypes.byref(ns), ctypes.byref(beta), nf)
         return {"ns": ns.value, "beta": beta.value, "nf": tuple(nf[: self.ndim])[::-1]}
+
+    def timing(self, stage, reset=False):
+        """(total ms, launches) of a stage; the plan must be made with debug=2."""
+        ms = c_double()
+        n = c_int64()
+        check(lib().jfa_plan_timing(self._plan, stage.encode(), int(reset), ctypes.byref(ms),
+                                    ctypes.byref(n)), "plan_timing")
+        return ms.value, n.value
 
     def _sync_stream(self):
         lib().jfa_set_stream(self._plan, torch.cuda.current_stream(self.device).cuda_stream)

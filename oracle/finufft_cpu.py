@@ -57,12 +57,16 @@ def phihat(ns, beta, nf, n):
 class Plan:
     """One FINUFFT-method problem (type 1 or 2) on the CPU, user axis order."""
 
-    def __init__(self, n_modes, eps, iflag, modeord=0, upsampfac=2.0, workers=None):
+    def __init__(self, n_modes, eps, iflag, modeord=0, upsampfac=2.0, workers=None, ns=None):
         self.n_modes = tuple(int(n) for n in n_modes)
         self.dim = len(self.n_modes)
         self.iflag = int(iflag)
         self.modeord = int(modeord)
         self.ns, self.beta, self.warn = params(eps, upsampfac)
+        if ns is not None and upsampfac == 2.0:
+            # a forced width, with the published beta/width ratio for it
+            self.ns = int(ns)
+            self.beta = {2: 2.20, 3: 2.26, 4: 2.38}.get(self.ns, 2.30) * self.ns
         self.nf_native = [fine_size(n, self.ns, upsampfac) for n in self.n_modes[::-1]]
         while len(self.nf_native) < 3:
             self.nf_native.append(1)

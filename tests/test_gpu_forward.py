@@ -128,6 +128,8 @@ def test_eps_bar_type1_type2(ndim, prec, dist, eps, method):
 @pytest.mark.parametrize("ndim,prec", product([1, 2, 3], ["f32", "f64"]))
 @pytest.mark.parametrize("extent", [1.0, 30.0])
 def test_eps_bar_type3(ndim, prec, extent):
+    if ndim == 3:
+        extent = min(extent, 4.0)  # X*S = 900 per axis in 3D needs a 2400^3 inner grid
     rng = np.random.default_rng(7 + ndim)
     M, N = 1500, 1300
     x = [rng.uniform(-extent, extent + 0.5, M).astype(NPR[prec]) for _ in range(ndim)]
@@ -175,7 +177,22 @@ def test_folding_of_points_outside_the_period(prec):
     b = host(nufft1((33, 20), dev(c), *[dev(v.astype(NPR[prec])) for v in shift], eps=1e-6))
     ref = oracle.type1([v.astype(NPR[prec]) for v in shift], c, (33, 20), 1)
     assert rel_l2(b, ref) <= 1e-6
-    assert rel_l2(a, b) < (1e-6 if prec == "f32" else 1e-12)
+    # in f32 the shifted coordinates are a different rounding of the same points
+    assert rel_l2(a, b) < (5e-5 if prec == "f32" else 1e-12)
+
+
+def test_oversized_type3_is_an_error_not_garbage():
+    from jax_finufft_amd._native import NufftError
+
+    rng = np.random.default_rng(8)
+    x = [dev(rng.uniform(-30, 30, 100)) for _ in range(3)]
+    s = [dev(rng.uniform(-30, 30, 100)) for _ in range(3)]
+    c = dev(cnormal(rng, 100, "f64"))
+    with pytest.raises(NufftError, match="exceed"):
+        nufft3(c, *x, *s, eps=1e-9)
+    # the library is still usable afterwards (no stale error state)
+    got = host(nufft1((8,), c, x[0], eps=1e-9))
+    assert rel_l2(got, oracle.type1([host(x[0])], host(c), (8,), 1)) < 1e-9
 
 
 @pytest.mark.parametrize("method", [1, 2])
