@@ -56,9 +56,12 @@ enum {
     JFA_ERR_TOO_LARGE = 7
 };
 
-/* Spread/interp method (gpu_method): 0 auto (autotuned), 1 global-memory
- * atomics in sorted order, 2 LDS subproblem tiles. */
-enum { JFA_METHOD_AUTO = 0, JFA_METHOD_GM_SORT = 1, JFA_METHOD_SM = 2 };
+/* Spread/interp method (gpu_method): 0 auto, 1 global-memory atomics in
+ * sorted order, 2 LDS subproblem tiles, 3 register strips (2D spreading:
+ * atomic-free; interpolation then uses the LDS tiles).  Auto picks 3 for 2D
+ * spreading on grids of at least 64 columns, else 2 when the tile fits in
+ * LDS, else 1. */
+enum { JFA_METHOD_AUTO = 0, JFA_METHOD_GM_SORT = 1, JFA_METHOD_SM = 2, JFA_METHOD_STRIP = 3 };
 
 /* Options.  The first seven fields are the ones the reference forwards to the
  * GPU (src/jax_finufft/options.py:105-119, lowering.py:337-343); the rest are

@@ -74,6 +74,18 @@ struct GridGeom {
     int64_t nbins;       // product of nb
     int ns;              // kernel width
     int dim;
+    int ymajor;          // 2D bins ordered y fastest (bin = by + nb1*bx), for strips
+};
+
+// Geometry of the 2D strip spreader (kernels_strip.hip): strips of SW = 64 -
+// ns + 1 columns, units of H rows, K units per strip.
+struct StripGeom {
+    int64_t nf0, nf1;
+    int SW, S;      // strip width (owned first taps) and strip count
+    int H, K;       // rows per unit, units per strip
+    int nunits;     // S * K
+    int ns;
+    int pmax;       // points per piece of a crowded unit
 };
 
 // Points after setpts, in bin-sorted order: original index, first tap
@@ -110,6 +122,11 @@ template <typename T>
 int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
                   const T *coef, int kerevalmeth, double beta, const T *c, int64_t c_stride,
                   T *fw, int64_t fw_stride, int ntr, hipStream_t stream);
+template <typename T>
+int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int *bin_start,
+                        int *piece_start, int *npieces, const T *coef, int kerevalmeth,
+                        double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
+                        T *scratch, int64_t scratch_stride, int ntr, hipStream_t stream);
 template <typename T>
 int launch_interp(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
                   const T *coef, int kerevalmeth, double beta, const T *fw, int64_t fw_stride,

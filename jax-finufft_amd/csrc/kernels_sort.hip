@@ -23,7 +23,10 @@ __device__ __forceinline__ int bin_of_point(const GridGeom &g, const T *x, const
     int b = f0.l0w / g.bs[0];
     if constexpr (DIM > 1) {
         Folded f1 = fold_point((double)y[j], g.nf[1], g.scale[1], g.shift[1], g.ns);
-        b += (f1.l0w / g.bs[1]) * g.nb[0];
+        if (DIM == 2 && g.ymajor)
+            b = b * g.nb[1] + f1.l0w / g.bs[1];
+        else
+            b += (f1.l0w / g.bs[1]) * g.nb[0];
     }
     if constexpr (DIM > 2) {
         Folded f2 = fold_point((double)z[j], g.nf[2], g.scale[2], g.shift[2], g.ns);

@@ -1,0 +1,281 @@
+// Atomic-free 2D spreading ("strip" method), the default type-1 spreader in 2D.
+//
+// Replaces the 2D type-1 spreading cuFINUFFT runs inside execute (called at
+// lib/kernels.cc.cu:75; source absent from the reference).  Designed around
+// two measured facts of gfx950: LDS float atomics are ~35x slower than LDS
+// integer atomics or plain LDS traffic, and coalesced plain stores are far
+// cheaper than global float atomics.
+//
+// Layout.  The fine grid is cut into vertical strips of SW = 64 - ns + 1
+// columns; strip s owns points whose first x-tap l0x lies in
+// [s*SW, (s+1)*SW), so all of their taps fall in the 64 columns
+// [s*SW, s*SW + 64).  Points are counting-sorted by (strip, l0y) (the
+// "y-major" bins set up by the plan), so within a strip they arrive in row
+// order.  A strip is cut into units of H consecutive l0y rows.
+//
+// One wave per unit: lane x owns column s*SW + x and holds a rolling window of
+// ns rows (real and imaginary parts) in registers.  Points are staged 64 at a
+// time: each lane evaluates one point's ns x-weights and ns y-weights (Horner)
+// and writes the x-weights and c*y-weights to wave-private LDS; then the wave
+// walks the batch in order, every lane reading its own x-weight (zero outside
+// the point's 8-wide window) and the point's y-weights by broadcast, and doing
+// 2*ns FMAs into its window.  When the next point starts a later row, the
+// window's first row is complete and is written out.  No atomics touch LDS.
+//
+// Output.  A unit's rows are written with plain coalesced stores into its own
+// scratch tile of (H + ns - 1) x 64 cells; a merge kernel then sums, for every
+// fine-grid cell, the (at most four) tiles that cover it.  A unit with more
+// than `pmax` points is split into pieces processed by separate waves; those
+// pieces add their rows to the (zeroed) fine grid with global atomics instead,
+// and the merge skips their tiles.
+#include "jfa_device.hpp"
+
+namespace jfa {
+namespace {
+
+constexpr int kStripLanes = 64;
+
+// unit -> (strip, chunk) and its point range; piece -> unit by binary search
+__device__ __forceinline__ int piece_unit(const int *__restrict__ piece_start, int nunits,
+                                          int piece) {
+    int lo = 0, hi = nunits;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (piece_start[mid] <= piece)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(kStripLanes)
+    spread_strip_kernel(StripGeom sg, SortedPts<T> pts, const int *__restrict__ bin_start,
+                        const int *__restrict__ piece_start, const int *__restrict__ npieces,
+                        const T *__restrict__ coef, T beta, int horner, const T *__restrict__ c,
+                        int64_t c_stride, T *__restrict__ fw, int64_t fw_stride,
+                        T *__restrict__ scratch, int64_t scratch_stride) {
+    __shared__ T s_kx[kStripLanes * NS];
+    __shared__ T s_cky[kStripLanes * NS * 2];
+    __shared__ int s_lx[kStripLanes];
+    __shared__ int s_ly[kStripLanes];
+    c += 2 * c_stride * blockIdx.y;
+    fw += 2 * fw_stride * blockIdx.y;
+    scratch += 2 * scratch_stride * blockIdx.y;
+    const int lane = threadIdx.x;
+    const int total = *npieces;
+    const int rows_tile = sg.H + NS - 1;
+    for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
+        const int u = piece_unit(piece_start, sg.nunits, piece);
+        const int sub = piece - piece_start[u];
+        const bool heavy = piece_start[u + 1] - piece_start[u] > 1;
+        const int s = u / sg.K, k = u - s * sg.K;
+        const int ox = s * sg.SW;
+        const int r0 = k * sg.H;
+        const int r1 = min(r0 + sg.H, (int)sg.nf1);
+        const int ub = bin_start[(int64_t)s * sg.nf1 + r0];
+        const int ue = bin_start[(int64_t)s * sg.nf1 + r1];
+        const int pb = heavy ? ub + sub * sg.pmax : ub;
+        const int pe = heavy ? min(pb + sg.pmax, ue) : ue;
+        T acc_r[NS], acc_i[NS];
+#pragma unroll
+        for (int b = 0; b < NS; ++b) acc_r[b] = acc_i[b] = T(0);
+        // rolling window: acc[b] holds row rcur + b
+        int rcur = heavy ? (pe > pb ? pts.l0[1][pb] : r0) : r0;
+        T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
+        auto flush = [&](int row) {
+            if (heavy) {
+                int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
+                int64_t x = ox + lane;
+                if (x >= sg.nf0) x -= sg.nf0;
+                const int64_t gi = y * sg.nf0 + x;
+                if (acc_r[0] != T(0) || acc_i[0] != T(0)) {
+                    atomic_add_global(&fw[2 * gi], acc_r[0]);
+                    atomic_add_global(&fw[2 * gi + 1], acc_i[0]);
+                }
+            } else {
+                const int64_t q = (int64_t)(row - r0) * kStripLanes + lane;
+                tile[2 * q] = acc_r[0];
+                tile[2 * q + 1] = acc_i[0];
+            }
+#pragma unroll
+            for (int b = 0; b + 1 < NS; ++b) {
+                acc_r[b] = acc_r[b + 1];
+                acc_i[b] = acc_i[b + 1];
+            }
+            acc_r[NS - 1] = acc_i[NS - 1] = T(0);
+        };
+        for (int bb = pb; bb < pe; bb += kStripLanes) {
+            const int nb = min(kStripLanes, pe - bb);
+            // stage: lane q evaluates point bb + q
+            if (lane < nb) {
+                const int p = bb + lane;
+                const int64_t j = pts.idx[p];
+                const T cr = c[2 * j], ci = c[2 * j + 1];
+                T kx[NS], ky[NS];
+                if (horner) {
+                    eval_kernel<T, NS, true>(coef, beta, pts.t[0][p], kx);
+                    eval_kernel<T, NS, true>(coef, beta, pts.t[1][p], ky);
+                } else {
+                    eval_kernel<T, NS, false>(coef, beta, pts.t[0][p], kx);
+                    eval_kernel<T, NS, false>(coef, beta, pts.t[1][p], ky);
+                }
+#pragma unroll
+                for (int a = 0; a < NS; ++a) s_kx[lane * NS + a] = kx[a];
+#pragma unroll
+                for (int b = 0; b < NS; ++b) {
+                    s_cky[(lane * NS + b) * 2] = cr * ky[b];
+                    s_cky[(lane * NS + b) * 2 + 1] = ci * ky[b];
+                }
+                s_lx[lane] = pts.l0[0][p] - ox;
+                s_ly[lane] = pts.l0[1][p];
+            }
+            __syncthreads();
+            for (int q = 0; q < nb; ++q) {
+                const int ly = s_ly[q];
+                while (rcur < ly) {
+                    flush(rcur);
+                    ++rcur;
+                }
+                const int kk = lane - s_lx[q];
+                const bool in = kk >= 0 && kk < NS;
+                const T kxv = in ? s_kx[q * NS + (in ? kk : 0)] : T(0);
+                const T *cy = &s_cky[q * NS * 2];
+#pragma unroll
+                for (int b = 0; b < NS; ++b) {
+                    acc_r[b] = fma(cy[2 * b], kxv, acc_r[b]);
+                    acc_i[b] = fma(cy[2 * b + 1], kxv, acc_i[b]);
+                }
+            }
+            __syncthreads();
+        }
+        // drain the window (and, for a full unit, every remaining row of its tile)
+        const int rstop = heavy ? rcur + NS : r0 + rows_tile;
+        while (rcur < rstop) {
+            flush(rcur);
+            ++rcur;
+        }
+    }
+}
+
+// fw[y][x] += sum of the scratch tiles of the light units covering (x, y).
+template <typename T, int NS>
+__global__ void __launch_bounds__(256)
+    strip_merge_kernel(StripGeom sg, const int *__restrict__ piece_start,
+                       const T *__restrict__ scratch, int64_t scratch_stride, T *__restrict__ fw,
+                       int64_t fw_stride) {
+    const int64_t F = sg.nf0 * sg.nf1;
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= F) return;
+    fw += 2 * fw_stride * blockIdx.y;
+    scratch += 2 * scratch_stride * blockIdx.y;
+    const int x = (int)(q % sg.nf0), y = (int)(q / sg.nf0);
+    const int rows_tile = sg.H + NS - 1;
+    // candidate strips: the one holding x, the previous one, and the last two
+    // (whose columns wrap past nf0)
+    int sc[4] = {x / sg.SW, x / sg.SW - 1, sg.S - 1, sg.S - 2};
+    int kc[4] = {y / sg.H, y / sg.H - 1, sg.K - 1, sg.K - 2};
+    T ar = fw[2 * q], ai = fw[2 * q + 1];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int s = sc[i];
+        bool dup = s < 0 || s >= sg.S;
+        for (int i2 = 0; i2 < i; ++i2) dup = dup || sc[i2] == s;
+        if (dup) continue;
+        int col = x - s * sg.SW;
+        if (col < 0) col += (int)sg.nf0;
+        if (col >= kStripLanes) continue;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int k = kc[jj];
+            bool dupk = k < 0 || k >= sg.K;
+            for (int j2 = 0; j2 < jj; ++j2) dupk = dupk || kc[j2] == k;
+            if (dupk) continue;
+            int row = y - k * sg.H;
+            if (row < 0) row += (int)sg.nf1;
+            if (row >= rows_tile) continue;
+            const int u = s * sg.K + k;
+            if (piece_start[u + 1] - piece_start[u] != 1) continue;  // heavy: already added
+            const int64_t t = ((int64_t)u * rows_tile + row) * kStripLanes + col;
+            ar += scratch[2 * t];
+            ai += scratch[2 * t + 1];
+        }
+    }
+    fw[2 * q] = ar;
+    fw[2 * q + 1] = ai;
+}
+
+// One workgroup: pieces per unit (1, or ceil(count / pmax) for a crowded
+// unit) and their exclusive prefix; total at [nunits] and in *npieces.
+__global__ void __launch_bounds__(1024)
+    strip_units_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
+                       int *npieces) {
+    __shared__ int sa[1024];
+    const int tid = threadIdx.x;
+    const int n = sg.nunits;
+    const int per = (n + 1023) / 1024;
+    const int lo = tid * per;
+    const int hi = min(lo + per, n);
+    auto pieces = [&](int u) {
+        const int s = u / sg.K, k = u - s * sg.K;
+        const int r0 = k * sg.H, r1 = min(r0 + sg.H, (int)sg.nf1);
+        const int cnt = bin_start[(int64_t)s * sg.nf1 + r1] - bin_start[(int64_t)s * sg.nf1 + r0];
+        return cnt <= sg.pmax ? 1 : (cnt + sg.pmax - 1) / sg.pmax;
+    };
+    int a = 0;
+    for (int u = lo; u < hi; ++u) a += pieces(u);
+    sa[tid] = a;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = tid >= off ? sa[tid - off] : 0;
+        __syncthreads();
+        sa[tid] += v;
+        __syncthreads();
+    }
+    int e = sa[tid] - a;
+    for (int u = lo; u < hi; ++u) {
+        piece_start[u] = e;
+        e += pieces(u);
+    }
+    if (tid == 1023) {
+        piece_start[n] = sa[1023];
+        *npieces = sa[1023];
+    }
+}
+
+}  // namespace
+
+template <typename T>
+int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int *bin_start,
+                        int *piece_start, int *npieces, const T *coef, int kerevalmeth,
+                        double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
+                        T *scratch, int64_t scratch_stride, int ntr, hipStream_t stream) {
+    if (ntr == 0) return JFA_OK;
+    strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces);
+    JFA_HIP_CHECK(hipGetLastError());
+    const int64_t ub = sg.nunits + pts.M / sg.pmax + 1;
+    const unsigned grid = (unsigned)(ub < 256 * 32 ? ub : 256 * 32);
+    const T b = (T)beta;
+    const int64_t F = sg.nf0 * sg.nf1;
+    JFA_DISPATCH_NS(sg.ns, {
+        spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
+            sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride, fw,
+            fw_stride, scratch, scratch_stride);
+        strip_merge_kernel<T, NS_><<<dim3((unsigned)((F + 255) / 256), ntr), 256, 0, stream>>>(
+            sg, piece_start, scratch, scratch_stride, fw, fw_stride);
+    });
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template int launch_spread_strip<float>(const StripGeom &, const SortedPts<float> &, const int *,
+                                        int *, int *, const float *, int, double, const float *,
+                                        int64_t, float *, int64_t, float *, int64_t, int,
+                                        hipStream_t);
+template int launch_spread_strip<double>(const StripGeom &, const SortedPts<double> &,
+                                         const int *, int *, int *, const double *, int, double,
+                                         const double *, int64_t, double *, int64_t, double *,
+                                         int64_t, int, hipStream_t);
+
+}  // namespace jfa

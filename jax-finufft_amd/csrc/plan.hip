@@ -88,6 +88,18 @@ size_t tile_lds_bytes(jfa_plan p) {
 int resolve_method(jfa_plan p) {
     const size_t kMaxTileLds = 64 * 1024;
     int m = p->opts.gpu_method;
+    const bool spreads = p->type != 2;
+    if (m == JFA_METHOD_STRIP || (m == JFA_METHOD_AUTO && spreads && strip_applicable(p))) {
+        if (spreads) {
+            if (!strip_applicable(p)) {
+                set_error("the strip method needs a 2D grid of at least 64 x %d cells",
+                          2 * p->geom.ns);
+                return -1;
+            }
+            return JFA_METHOD_STRIP;
+        }
+        m = JFA_METHOD_SM;  // interpolation has no strip variant
+    }
     if (m == JFA_METHOD_AUTO) m = tile_lds_bytes(p) <= kMaxTileLds ? JFA_METHOD_SM : JFA_METHOD_GM_SORT;
     // an explicit LDS request with a wide kernel: halve the largest bin until
     // the tile fits
@@ -134,6 +146,62 @@ struct DeviceGuard {
 };
 
 }  // namespace
+
+// ------------------------------------------------------------------ strips
+
+bool jfa::strip_applicable(jfa_plan p) {
+    const GridGeom &g = p->geom;
+    return p->dim == 2 && g.nf[0] >= 64 && g.nf[1] >= 2 * g.ns;
+}
+
+// Strip geometry and bins for the 2D strip spreader, and its buffers.  Units
+// of H rows are sized for about 4096 units (a few waves per SIMD), with H in
+// [ns, nf1 - ns + 1] so that a cell is covered by at most two units per axis.
+int jfa::setup_strip(jfa_plan p) {
+    GridGeom &g = p->geom;
+    StripGeom &sg = p->sg;
+    const int ns = g.ns;
+    sg.nf0 = g.nf[0];
+    sg.nf1 = g.nf[1];
+    sg.ns = ns;
+    sg.SW = 64 - ns + 1;
+    sg.S = (int)((g.nf[0] + sg.SW - 1) / sg.SW);
+    int64_t H = (g.nf[1] * sg.S + 4095) / 4096;
+    if (H < 16) H = 16;
+    if (H > g.nf[1] - ns + 1) H = g.nf[1] - ns + 1;
+    if (H < ns) H = ns;
+    sg.H = (int)H;
+    sg.K = (int)((g.nf[1] + H - 1) / H);
+    sg.nunits = sg.S * sg.K;
+    sg.pmax = p->opts.gpu_maxsubprobsize > 0 ? p->opts.gpu_maxsubprobsize : 4096;
+    // bins: (strip, exact row), rows fastest
+    g.bs[0] = sg.SW;
+    g.bs[1] = 1;
+    g.bs[2] = 1;
+    g.nb[0] = sg.S;
+    g.nb[1] = (int)g.nf[1];
+    g.nb[2] = 1;
+    g.nbins = (int64_t)g.nb[0] * g.nb[1];
+    g.ymajor = 1;
+    // buffers
+    free_ptr(reinterpret_cast<void *&>(p->d_piece_start));
+    JFA_HIP_CHECK(hipMalloc(&p->d_piece_start, sizeof(int) * (sg.nunits + 1)));
+    if (!p->d_npieces) JFA_HIP_CHECK(hipMalloc(&p->d_npieces, sizeof(int) * 4));
+    const int64_t rows_tile = sg.H + ns - 1;
+    p->strip_scratch_stride = (int64_t)sg.nunits * rows_tile * 64;
+    const size_t bytes =
+        (size_t)p->batch * p->strip_scratch_stride * 2 * (p->is_double ? 8 : 4);
+    if (bytes > p->strip_scratch_bytes) {
+        free_ptr(p->d_strip_scratch);
+        if (hipMalloc(&p->d_strip_scratch, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("could not allocate %zu bytes of strip scratch", bytes);
+            return JFA_ERR_ALLOC;
+        }
+        p->strip_scratch_bytes = bytes;
+    }
+    return JFA_OK;
+}
 
 // ------------------------------------------------------------------ fft
 
@@ -412,6 +480,12 @@ extern "C" int jfa_makeplan(int type, int dim, const int64_t *nmodes, int iflag,
             jfa_destroy(p);
             return JFA_ERR_BAD_ARG;
         }
+        if (p->method == JFA_METHOD_STRIP) {
+            if (int rc = setup_strip(p)) {
+                jfa_destroy(p);
+                return rc;
+            }
+        }
         int rc = upload_kernel_tables(p, p->nmodes);
         if (rc) {
             jfa_destroy(p);
@@ -478,7 +552,8 @@ int jfa::reserve_points(jfa_plan p, int64_t M) {
 template <typename T>
 static int sort_points(jfa_plan p, int64_t M, const T *x, const T *y, const T *z) {
     const GridGeom &g = p->geom;
-    const int sorted = (p->method == JFA_METHOD_SM || p->opts.gpu_sort) ? 1 : 0;
+    const int sorted =
+        (p->method == JFA_METHOD_SM || p->method == JFA_METHOD_STRIP || p->opts.gpu_sort) ? 1 : 0;
     int rc;
     if (sorted) {
         if ((rc = launch_fill_zero(p->d_bin_count, sizeof(int) * (g.nbins + 1), p->stream)))
@@ -555,6 +630,29 @@ BinIndex jfa::bin_view(jfa_plan p) {
     return b;
 }
 
+// Spread ntr transforms (strengths c with stride c_stride, grids fw with
+// stride fw_stride, fw already zeroed) with the plan's method.
+template <typename T>
+int jfa::spread_any(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t fw_stride, int ntr) {
+    const SortedPts<T> pts = sorted_view<T>(p);
+    const int kem = p->opts.gpu_kerevalmeth ? 1 : 0;
+    if (p->method != JFA_METHOD_STRIP)
+        return launch_spread<T>(p->method, p->geom, pts, bin_view(p), (const T *)p->d_coef, kem,
+                                p->kp.beta, c, c_stride, fw, fw_stride, ntr, p->stream);
+    for (int t0 = 0; t0 < ntr; t0 += p->batch) {
+        const int nb = std::min(p->batch, ntr - t0);
+        int rc = launch_spread_strip<T>(p->sg, pts, p->d_bin_start, p->d_piece_start,
+                                        p->d_npieces, (const T *)p->d_coef, kem, p->kp.beta,
+                                        c + 2 * t0 * c_stride, c_stride, fw + 2 * t0 * fw_stride,
+                                        fw_stride, (T *)p->d_strip_scratch,
+                                        p->strip_scratch_stride, nb, p->stream);
+        if (rc) return rc;
+    }
+    return JFA_OK;
+}
+template int jfa::spread_any<float>(jfa_plan, const float *, int64_t, float *, int64_t, int);
+template int jfa::spread_any<double>(jfa_plan, const double *, int64_t, double *, int64_t, int);
+
 // Type 1 / type 2 execution on the plan's current points.  c has ntransf*M
 // complex values, fk ntransf*Ntot.
 template <typename T>
@@ -574,8 +672,7 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
             if ((rc = launch_fill_zero(fk, sizeof(T) * 2 * p->F * p->ntransf, p->stream)))
                 return rc;
             tm.mark("zero");
-            rc = launch_spread<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, c, M, fk, p->F,
-                                  p->ntransf, p->stream);
+            rc = spread_any<T>(p, c, M, fk, p->F, p->ntransf);
             tm.mark("spread");
         } else {
             rc = launch_interp<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, fk, p->F, c, M,
@@ -593,9 +690,7 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
         if (p->type == 1) {
             if ((rc = launch_fill_zero(fw, sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
             tm.mark("zero");
-            if ((rc = launch_spread<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, cb, M, fw,
-                                       p->F, nb, p->stream)))
-                return rc;
+            if ((rc = spread_any<T>(p, cb, M, fw, p->F, nb))) return rc;
             tm.mark("spread");
             if ((rc = fft_exec(p, fw, nb))) return rc;
             tm.mark("fft");
@@ -680,7 +775,8 @@ extern "C" int jfa_destroy(jfa_plan p) {
     free_ptr(p->d_fw);
     for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_bin_of,
                      (void **)&p->d_rank_of, &p->d_t, (void **)&p->d_bin_count,
-                     (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub})
+                     (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub,
+                     (void **)&p->d_piece_start, (void **)&p->d_npieces, &p->d_strip_scratch})
         free_ptr(*q);
     jfa::type3_free(p);
     for (hipEvent_t e : p->timing_events) (void)hipEventDestroy(e);

@@ -49,6 +49,13 @@ struct jfa_plan_s {
     void *d_t = nullptr;
     int *d_bin_count = nullptr, *d_bin_start = nullptr, *d_sub_start = nullptr, *d_nsub = nullptr;
 
+    // 2D strip spreader
+    jfa::StripGeom sg{};
+    int *d_piece_start = nullptr, *d_npieces = nullptr;
+    void *d_strip_scratch = nullptr;
+    size_t strip_scratch_bytes = 0;
+    int64_t strip_scratch_stride = 0;  // complex cells per transform
+
     // type 3
     jfa_plan inner = nullptr;     // type-2 plan from the spread grid to the targets
     const void *t3_x[3] = {nullptr, nullptr, nullptr};   // user source coordinates
@@ -67,6 +74,8 @@ struct jfa_plan_s {
 namespace jfa {
 
 int fft_exec(jfa_plan p, void *data, int nbatch);
+bool strip_applicable(jfa_plan p);
+int setup_strip(jfa_plan p);
 int upload_kernel_tables(jfa_plan p, const int64_t *nmodes_for_dk);
 int default_maxsub(jfa_plan p);
 int reserve_points(jfa_plan p, int64_t M);
@@ -76,6 +85,8 @@ SortedPts<T> sorted_view(jfa_plan p);
 BinIndex bin_view(jfa_plan p);
 template <typename T>
 int execute_12(jfa_plan p, T *c, T *fk);
+template <typename T>
+int spread_any(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t fw_stride, int ntr);
 
 int type3_init(jfa_plan p);
 int type3_setpts(jfa_plan p, int64_t M, const void *x, const void *y, const void *z, int64_t N,

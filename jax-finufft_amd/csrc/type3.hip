@@ -170,8 +170,15 @@ static int type3_setpts_t(jfa_plan p, int64_t M, const T *const xs[3], int64_t N
         size_t cells = 1;
         for (int d = 0; d < dim; ++d) cells *= (size_t)(g.bs[d] + ns - 1);
         const bool fits = cells * 2 * sizeof(T) <= 64 * 1024;
+        g.ymajor = 0;
         p->method = p->opts.gpu_method == JFA_METHOD_GM_SORT || !fits ? JFA_METHOD_GM_SORT
                                                                       : JFA_METHOD_SM;
+        const int om = p->opts.gpu_method;
+        if ((om == JFA_METHOD_AUTO || om == JFA_METHOD_STRIP) && strip_applicable(p)) {
+            int rc = setup_strip(p);
+            if (rc) return rc;
+            p->method = JFA_METHOD_STRIP;
+        }
     }
     // fine grid and prephase buffers
     if (p->F * p->batch > p->t3_nf_cap) {
@@ -288,9 +295,7 @@ static int type3_execute_t(jfa_plan p, const T *c, T *f) {
             src = (const T *)p->d_cp;
         }
         if ((rc = launch_fill_zero(fw, (int64_t)sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
-        if ((rc = launch_spread<T>(p->method, p->geom, pts, bi, (const T *)p->d_coef, kem,
-                                   p->kp.beta, src, M, fw, p->F, nb, p->stream)))
-            return rc;
+        if ((rc = spread_any<T>(p, src, M, fw, p->F, nb))) return rc;
         const int saved = p->inner->ntransf;
         p->inner->ntransf = nb;
         rc = execute_12<T>(p->inner, fb, fw);
