@@ -103,6 +103,17 @@ int jfa_setpts(jfa_plan plan, int64_t M, const void *x, const void *y, const voi
  * type 3: c (ntransf x M) -> fk (ntransf x N). */
 int jfa_execute(jfa_plan plan, void *c, void *fk);
 
+/* Fused adjoint pass of a type-2 plan (no reference counterpart; it replaces
+ * the D extra stacked transforms of the reference's differentiation rule,
+ * src/jax_finufft/ops.py:214-236 and :295-312).  From the mode array fk
+ * (ntransf x modes) it computes, in one interpolation pass sharing one
+ * amplify + FFT:
+ *   val[t][j]   = sum_k fk[t][k] e^{i iflag k.x_j}            (if val != NULL)
+ *   grad[d][j]  = sum_t Re(conj(w[t][j]) d val[t][j] / d x_dj)  (if grad != NULL)
+ * with d in the plan's x-fastest order and grad laid out d-major (dim x M).
+ * Needs the LDS method (JFA_ERR_NOT_SUPPORTED otherwise). */
+int jfa_interp_grad(jfa_plan plan, const void *fk, const void *w, void *val, void *grad);
+
 /* Queue subsequent work of this plan on `stream`, ordered after the plan's
  * previous work on whatever stream it used before. */
 int jfa_set_stream(jfa_plan plan, jfa_stream_t stream);
@@ -202,6 +213,28 @@ int jfa_nufft3d3f(JFA_ATTRS_F, const void *source, const void *points_x, const v
 int jfa_nufft3d3(JFA_ATTRS_D, const void *source, const void *points_x, const void *points_y,
                  const void *points_z, const void *target_x, const void *target_y,
                  const void *target_z, void *output);
+
+/* Fused type-2 adjoint entry points (no reference counterpart): the same
+ * attributes and plan cache as jfa_nufft{d}d2{f}, then jfa_interp_grad per
+ * problem.  weights and value_out are (n_tot, n_transf, n_j) complex,
+ * grad_out is (n_tot, dim, n_j) real in x-fastest dimension order; value_out
+ * or grad_out (with weights) may be NULL. */
+int jfa_nufft1d2f_grad(JFA_ATTRS_F, const void *source, const void *points_x,
+                       const void *weights, void *value_out, void *grad_out);
+int jfa_nufft1d2_grad(JFA_ATTRS_D, const void *source, const void *points_x,
+                      const void *weights, void *value_out, void *grad_out);
+int jfa_nufft2d2f_grad(JFA_ATTRS_F, const void *source, const void *points_x,
+                       const void *points_y, const void *weights, void *value_out,
+                       void *grad_out);
+int jfa_nufft2d2_grad(JFA_ATTRS_D, const void *source, const void *points_x,
+                      const void *points_y, const void *weights, void *value_out,
+                      void *grad_out);
+int jfa_nufft3d2f_grad(JFA_ATTRS_F, const void *source, const void *points_x,
+                       const void *points_y, const void *points_z, const void *weights,
+                       void *value_out, void *grad_out);
+int jfa_nufft3d2_grad(JFA_ATTRS_D, const void *source, const void *points_x,
+                      const void *points_y, const void *points_z, const void *weights,
+                      void *value_out, void *grad_out);
 
 #ifdef __cplusplus
 }

@@ -186,7 +186,94 @@ int run(int type, int dim, jfa_stream_t stream, double eps, int64_t iflag, int64
     return JFA_OK;
 }
 
+// The fused type-2 adjoint: same plan (cache key of the type-2 transform),
+// then jfa_interp_grad per problem instead of execute.
+template <typename T>
+int run_grad(int dim, jfa_stream_t stream, double eps, int64_t iflag, int64_t n_tot,
+             int64_t n_transf, int64_t n_j, int64_t n_k_1, int64_t n_k_2, int64_t n_k_3,
+             int64_t modeord, double upsampfac, int64_t gpu_method, int64_t gpu_sort,
+             int64_t gpu_kerevalmeth, int64_t gpu_maxbatchsize, int64_t debug,
+             const void *source, const void *const pts[3], const void *weights, void *value_out,
+             void *grad_out) {
+    if (n_tot < 0 || n_transf < 1 || n_j < 0 || n_transf > INT32_MAX) {
+        set_error("bad sizes: n_tot=%lld n_transf=%lld n_j=%lld", (long long)n_tot,
+                  (long long)n_transf, (long long)n_j);
+        return JFA_ERR_BAD_ARG;
+    }
+    const int64_t nk[3] = {n_k_1, n_k_2, n_k_3};
+    Key k;
+    std::memset(&k, 0, sizeof(k));
+    if (hipGetDevice(&k.device) != hipSuccess) k.device = 0;
+    k.type = 2;
+    k.dim = dim;
+    k.is_double = sizeof(T) == 8;
+    k.iflag = iflag >= 0 ? 1 : -1;
+    k.ntransf = (int)n_transf;
+    int64_t n_k_total = 1;
+    for (int d = 0; d < 3; ++d) {
+        k.n[d] = d < dim ? nk[d] : 1;
+        if (d < dim && nk[d] < 1) {
+            set_error("n_k_%d must be >= 1", d + 1);
+            return JFA_ERR_BAD_ARG;
+        }
+        n_k_total *= k.n[d];
+    }
+    k.eps = eps;
+    k.upsampfac = upsampfac;
+    k.modeord = (int)modeord;
+    k.gpu_method = (int)gpu_method;
+    k.gpu_sort = (int)gpu_sort;
+    k.kerevalmeth = (int)gpu_kerevalmeth;
+    k.maxbatch = (int)gpu_maxbatchsize;
+    k.debug = (int)debug;
+    if (n_tot == 0) return JFA_OK;
+    jfa_opts o;
+    jfa_default_opts(&o);
+    o.modeord = k.modeord;
+    o.upsampfac = upsampfac;
+    o.gpu_method = k.gpu_method;
+    o.gpu_sort = k.gpu_sort;
+    o.gpu_kerevalmeth = k.kerevalmeth;
+    o.gpu_maxbatchsize = k.maxbatch;
+    o.debug = k.debug;
+    o.gpu_stream = stream;
+    int rc = JFA_OK;
+    jfa_plan p = checkout(k, o, &rc);
+    if (!p) return rc;
+    rc = jfa_set_stream(p, stream);
+    const size_t cplx = 2 * sizeof(T);
+    for (int64_t idx = 0; idx < n_tot && rc == JFA_OK; ++idx) {
+        const int64_t i = idx * n_j;
+        const int64_t j = idx * n_j * n_transf;
+        const int64_t kk = idx * n_k_total * n_transf;
+        const T *x = (const T *)pts[0] + i;
+        const T *y = dim > 1 ? (const T *)pts[1] + i : nullptr;
+        const T *z = dim > 2 ? (const T *)pts[2] + i : nullptr;
+        rc = jfa_setpts(p, n_j, x, y, z, 0, nullptr, nullptr, nullptr);
+        if (rc) break;
+        rc = jfa_interp_grad(p, (const char *)source + cplx * kk,
+                             weights ? (const char *)weights + cplx * j : nullptr,
+                             value_out ? (char *)value_out + cplx * j : nullptr,
+                             grad_out ? (char *)grad_out + sizeof(T) * dim * i : nullptr);
+    }
+    checkin(p);
+    if (rc) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("HIP error: %s", hipGetErrorString(e));
+        return JFA_ERR_HIP;
+    }
+    return JFA_OK;
+}
+
 }  // namespace
+
+#define JFA_DEFINE_GRAD(NAME, ATTRS, T, DIM, ...)                                            \
+    extern "C" int NAME(ATTRS, const void *source, __VA_ARGS__, const void *weights,          \
+                        void *value_out, void *grad_out) {                                    \
+        const void *const pts[3] = {JFA_PTS_##DIM};                                           \
+        return run_grad<T>(DIM, JFA_FWD_ATTRS, source, pts, weights, value_out, grad_out);    \
+    }
 
 extern "C" int jfa_cache_size(void) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -260,3 +347,10 @@ JFA_DEFINE_3(jfa_nufft2d3f, JFA_ATTRS_F, float, 2, P2, T2)
 JFA_DEFINE_3(jfa_nufft2d3, JFA_ATTRS_D, double, 2, P2, T2)
 JFA_DEFINE_3(jfa_nufft3d3f, JFA_ATTRS_F, float, 3, P3, T3)
 JFA_DEFINE_3(jfa_nufft3d3, JFA_ATTRS_D, double, 3, P3, T3)
+
+JFA_DEFINE_GRAD(jfa_nufft1d2f_grad, JFA_ATTRS_F, float, 1, P1)
+JFA_DEFINE_GRAD(jfa_nufft1d2_grad, JFA_ATTRS_D, double, 1, P1)
+JFA_DEFINE_GRAD(jfa_nufft2d2f_grad, JFA_ATTRS_F, float, 2, P2)
+JFA_DEFINE_GRAD(jfa_nufft2d2_grad, JFA_ATTRS_D, double, 2, P2)
+JFA_DEFINE_GRAD(jfa_nufft3d2f_grad, JFA_ATTRS_F, float, 3, P3)
+JFA_DEFINE_GRAD(jfa_nufft3d2_grad, JFA_ATTRS_D, double, 3, P3)

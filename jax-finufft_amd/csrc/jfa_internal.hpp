@@ -106,16 +106,25 @@ struct BinIndex {
     int maxsub;
 };
 
+// Scratch of the point sort: keys/values for the radix sort and its temp.
+struct SortScratch {
+    unsigned *key_in, *val_in, *key_out;
+    void *temp;
+    size_t temp_bytes;
+};
+
 // Launchers (kernels_*.hip).  All queue work on `stream` and return a JFA code.
+size_t sort_temp_bytes(int64_t M);
+// Bin, radix-sort and gather M points: idx/l0/tv in bin order, bin_start
+// (nbins + 1) and, when sub_start is non-null, the LDS-subproblem prefix.
 template <typename T>
-int launch_fold_bin(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
-                    int *bin_of, int *rank_of, int *bin_count, hipStream_t stream);
-int launch_bin_scan(const GridGeom &g, const int *bin_count, int *bin_start, int *sub_start,
-                    int *nsub, int maxsub, hipStream_t stream);
+int launch_sort_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                       const SortScratch &s, int *bin_start, int *sub_start, int *nsub,
+                       int maxsub, int *idx, int *l0, T *tv, hipStream_t stream);
+// Same per-point data in input order (no sort).
 template <typename T>
-int launch_scatter(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
-                   const int *bin_of, const int *rank_of, const int *bin_start, int sorted,
-                   int *idx, int *l0, T *tv, hipStream_t stream);
+int launch_fold_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                       int *idx, int *l0, T *tv, hipStream_t stream);
 int launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
 
 template <typename T>
@@ -131,6 +140,12 @@ template <typename T>
 int launch_interp(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
                   const T *coef, int kerevalmeth, double beta, const T *fw, int64_t fw_stride,
                   T *c, int64_t c_stride, int ntr, hipStream_t stream);
+// Fused interpolation + point gradient (LDS method); grad[d*grad_stride + j].
+template <typename T>
+int launch_interp_grad(const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
+                       const T *coef, const T *fw, int64_t fw_stride, int ntr, const T *w,
+                       int64_t w_stride, T *val, int64_t val_stride, T *grad,
+                       int64_t grad_stride, int accumulate, hipStream_t stream);
 template <typename T>
 int launch_deconv(int dim, const int64_t *n, const int64_t *nf, const T *dk0, const T *dk1,
                   const T *dk2, int modeord, const T *fw, int64_t fw_stride, T *out,

@@ -1,155 +1,139 @@
-// setpts: fold points onto the fine grid, bin them, and counting-sort them
-// into bin order.  Replaces the point sort cuFINUFFT runs inside setpts
-// (called at lib/kernels.cc.cu:63-68; device code absent from the reference).
+// setpts: fold points onto the fine grid, bin them and sort them by bin.
+// Replaces the point sort cuFINUFFT runs inside setpts (called at
+// lib/kernels.cc.cu:63-68; device code absent from the reference).
 //
-// Bin of a point = bin of its first tap l0 (wrapped), so every tap of a point
-// in bin b lies in the tile [origin(b), origin(b) + bs + ns - 1).  The integer
-// tap position is produced by fold_point() from the coordinate in double, and
-// the same function is called by the binning and by the scatter pass, so the
-// bin a point is counted in and the tap window the spreader uses agree exactly.
+//   1. keys:   key[j] = bin of point j (from its first tap), val[j] = j
+//   2. sort:   rocPRIM radix sort of (key, val) pairs over the bits the bin
+//              count needs (no atomics, so clustered inputs cost the same)
+//   3. bounds: bin_start[b] = first sorted position with key >= b (binary
+//              search per bin), and for the LDS method the subproblem prefix
+//   4. gather: for sorted position p, j = val[p]: write the point's wrapped
+//              first tap and Horner variable per dimension at p (coalesced
+//              writes; the coordinates are read by index)
+//
+// The bin of a point and the tap window the spreader uses both come from
+// fold_point() on the coordinate in double, so they agree exactly.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "jfa_device.hpp"
 
 namespace jfa {
 
 namespace {
 
-constexpr int kBinThreads = 256;
-constexpr int kBinPPT = 16;  // points per thread in the LDS-histogram pass
-
 template <typename T, int DIM>
-__device__ __forceinline__ int bin_of_point(const GridGeom &g, const T *x, const T *y,
-                                            const T *z, int64_t j) {
+__device__ __forceinline__ unsigned bin_of_point(const GridGeom &g, const T *x, const T *y,
+                                                 const T *z, int64_t j) {
     Folded f0 = fold_point((double)x[j], g.nf[0], g.scale[0], g.shift[0], g.ns);
-    int b = f0.l0w / g.bs[0];
+    int64_t b = f0.l0w / g.bs[0];
     if constexpr (DIM > 1) {
         Folded f1 = fold_point((double)y[j], g.nf[1], g.scale[1], g.shift[1], g.ns);
         if (DIM == 2 && g.ymajor)
             b = b * g.nb[1] + f1.l0w / g.bs[1];
         else
-            b += (f1.l0w / g.bs[1]) * g.nb[0];
+            b += (int64_t)(f1.l0w / g.bs[1]) * g.nb[0];
     }
     if constexpr (DIM > 2) {
         Folded f2 = fold_point((double)z[j], g.nf[2], g.scale[2], g.shift[2], g.ns);
-        b += (f2.l0w / g.bs[2]) * (g.nb[0] * g.nb[1]);
+        b += (int64_t)(f2.l0w / g.bs[2]) * g.nb[0] * g.nb[1];
     }
-    return b;
+    return (unsigned)b;
 }
 
-// Direct version: one global atomic per point (used when the bin histogram
-// does not fit in LDS).
 template <typename T, int DIM>
-__global__ void __launch_bounds__(kBinThreads)
-    fold_bin_global_kernel(GridGeom g, int64_t M, const T *__restrict__ x,
-                           const T *__restrict__ y, const T *__restrict__ z, int *bin_of,
-                           int *rank_of, int *bin_count) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+    bin_keys_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
+                    const T *__restrict__ z, unsigned *key, unsigned *val) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= M) return;
-    int b = bin_of_point<T, DIM>(g, x, y, z, j);
-    bin_of[j] = b;
-    rank_of[j] = atomicAdd(&bin_count[b], 1);
+    key[j] = bin_of_point<T, DIM>(g, x, y, z, j);
+    val[j] = (unsigned)j;
 }
 
-// LDS-histogram version: ranks within the block come from LDS atomics, then
-// one global atomic per (block, non-empty bin) reserves the block's slice of
-// each bin.  Clustered inputs hit a handful of bins; this keeps their global
-// atomic traffic to one per block instead of one per point.
-template <typename T, int DIM>
-__global__ void __launch_bounds__(kBinThreads)
-    fold_bin_lds_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
-                        const T *__restrict__ z, int *bin_of, int *rank_of, int *bin_count) {
-    extern __shared__ int hist[];
-    const int nbins = (int)g.nbins;
-    for (int b = threadIdx.x; b < nbins; b += kBinThreads) hist[b] = 0;
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * (kBinThreads * kBinPPT);
-    int bins[kBinPPT], ranks[kBinPPT];
-#pragma unroll
-    for (int k = 0; k < kBinPPT; ++k) {
-        int64_t j = base + (int64_t)k * kBinThreads + threadIdx.x;
-        bins[k] = -1;
-        if (j < M) {
-            bins[k] = bin_of_point<T, DIM>(g, x, y, z, j);
-            ranks[k] = atomicAdd(&hist[bins[k]], 1);
-        }
+// bin_start[b] = first p with skey[p] >= b, for b in [0, nbins]
+__global__ void __launch_bounds__(256)
+    bin_bounds_kernel(int64_t M, int64_t nbins, const unsigned *__restrict__ skey,
+                      int *bin_start) {
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b > nbins) return;
+    int64_t lo = 0, hi = M;  // answer in [lo, hi]
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)skey[mid] < b)
+            lo = mid + 1;
+        else
+            hi = mid;
     }
-    __syncthreads();
-    for (int b = threadIdx.x; b < nbins; b += kBinThreads) {
-        int h = hist[b];
-        hist[b] = h ? atomicAdd(&bin_count[b], h) : 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kBinPPT; ++k) {
-        int64_t j = base + (int64_t)k * kBinThreads + threadIdx.x;
-        if (bins[k] >= 0) {
-            bin_of[j] = bins[k];
-            rank_of[j] = ranks[k] + hist[bins[k]];
-        }
-    }
+    bin_start[b] = (int)lo;
 }
 
-// One workgroup: exclusive prefix sums of the bin counts and of the number of
-// subproblems per bin (ceil(count / maxsub)); writes the totals at [nbins] and
-// the subproblem total to *nsub.
+// One workgroup: exclusive prefix of the subproblems per bin
+// (ceil(count / maxsub)); total at sub_start[nbins] and *nsub.
 __global__ void __launch_bounds__(1024)
-    bin_scan_kernel(int64_t nbins, const int *__restrict__ count, int *bin_start,
-                    int *sub_start, int *nsub, int maxsub) {
+    sub_scan_kernel(int64_t nbins, const int *__restrict__ bin_start, int *sub_start, int *nsub,
+                    int maxsub) {
     __shared__ int sa[1024];
-    __shared__ int sb[1024];
     const int tid = threadIdx.x;
     const int64_t per = (nbins + 1023) / 1024;
     const int64_t lo = tid * per;
     const int64_t hi = lo + per < nbins ? lo + per : nbins;
-    int a = 0, s = 0;
+    int s = 0;
     for (int64_t i = lo; i < hi; ++i) {
-        int cnt = count[i];
-        a += cnt;
+        const int cnt = bin_start[i + 1] - bin_start[i];
         s += (cnt + maxsub - 1) / maxsub;
     }
-    sa[tid] = a;
-    sb[tid] = s;
+    sa[tid] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        int va = tid >= off ? sa[tid - off] : 0;
-        int vb = tid >= off ? sb[tid - off] : 0;
+        int v = tid >= off ? sa[tid - off] : 0;
         __syncthreads();
-        sa[tid] += va;
-        sb[tid] += vb;
+        sa[tid] += v;
         __syncthreads();
     }
-    int ea = sa[tid] - a, eb = sb[tid] - s;
+    int e = sa[tid] - s;
     for (int64_t i = lo; i < hi; ++i) {
-        int cnt = count[i];
-        bin_start[i] = ea;
-        sub_start[i] = eb;
-        ea += cnt;
-        eb += (cnt + maxsub - 1) / maxsub;
+        sub_start[i] = e;
+        const int cnt = bin_start[i + 1] - bin_start[i];
+        e += (cnt + maxsub - 1) / maxsub;
     }
     if (tid == 1023) {
-        bin_start[nbins] = sa[1023];
-        sub_start[nbins] = sb[1023];
-        *nsub = sb[1023];
+        sub_start[nbins] = sa[1023];
+        *nsub = sa[1023];
     }
 }
 
-// Writes each point's original index, wrapped first tap and Horner variable at
-// its sorted position (or in input order when `sorted` is 0).
+// Sorted position p holds point j = idx[p] (written by the sort): store its
+// wrapped first tap and Horner variable per dimension at p.
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256)
-    scatter_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
-                   const T *__restrict__ z, const int *__restrict__ bin_of,
-                   const int *__restrict__ rank_of, const int *__restrict__ bin_start,
-                   int sorted, int *idx, int *l0, T *tv) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
-    int64_t pos = sorted ? (int64_t)bin_start[bin_of[j]] + rank_of[j] : j;
-    idx[pos] = (int)j;
+    gather_points_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
+                         const T *__restrict__ z, const int *__restrict__ idx, int *l0, T *tv) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= M) return;
+    const int64_t j = idx[p];
     const T *xs[3] = {x, y, z};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         Folded f = fold_point((double)xs[d][j], g.nf[d], g.scale[d], g.shift[d], g.ns);
-        l0[d * M + pos] = f.l0w;
-        tv[d * M + pos] = (T)f.t;
+        l0[d * M + p] = f.l0w;
+        tv[d * M + p] = (T)f.t;
+    }
+}
+
+// Unsorted variant (global-memory method with gpu_sort = 0): input order.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+    fold_points_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
+                       const T *__restrict__ z, int *idx, int *l0, T *tv) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= M) return;
+    idx[p] = (int)p;
+    const T *xs[3] = {x, y, z};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        Folded f = fold_point((double)xs[d][p], g.nf[d], g.scale[d], g.shift[d], g.ns);
+        l0[d * M + p] = f.l0w;
+        tv[d * M + p] = (T)f.t;
     }
 }
 
@@ -158,6 +142,12 @@ __global__ void fill_zero_kernel(uint4 *p, int64_t n16, unsigned char *tail, int
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
     if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+int key_bits(int64_t nbins) {
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < nbins) ++bits;
+    return bits;
 }
 
 }  // namespace
@@ -179,54 +169,69 @@ int launch_fill_zero(void *p, int64_t bytes, hipStream_t stream) {
     return JFA_OK;
 }
 
-template <typename T>
-int launch_fold_bin(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
-                    int *bin_of, int *rank_of, int *bin_count, hipStream_t stream) {
-    if (M == 0) return JFA_OK;
-    const size_t lds = sizeof(int) * (size_t)g.nbins;
-    if (g.nbins <= 8192) {
-        int64_t per_block = (int64_t)kBinThreads * kBinPPT;
-        unsigned blocks = (unsigned)((M + per_block - 1) / per_block);
-        JFA_DISPATCH_DIM(g.dim, fold_bin_lds_kernel<T, DIM_><<<blocks, kBinThreads, lds, stream>>>(
-                                    g, M, x, y, z, bin_of, rank_of, bin_count));
-    } else {
-        unsigned blocks = (unsigned)((M + kBinThreads - 1) / kBinThreads);
-        JFA_DISPATCH_DIM(g.dim, fold_bin_global_kernel<T, DIM_><<<blocks, kBinThreads, 0, stream>>>(
-                                    g, M, x, y, z, bin_of, rank_of, bin_count));
+size_t sort_temp_bytes(int64_t M) {
+    size_t bytes = 0;
+    const unsigned *k = nullptr;
+    unsigned *ko = nullptr;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, k, ko, k, ko, (unsigned)(M > 0 ? M : 1), 0, 32,
+                                  (hipStream_t)0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
     }
-    JFA_HIP_CHECK(hipGetLastError());
-    return JFA_OK;
+    return bytes;
 }
 
-int launch_bin_scan(const GridGeom &g, const int *bin_count, int *bin_start, int *sub_start,
-                    int *nsub, int maxsub, hipStream_t stream) {
-    bin_scan_kernel<<<1, 1024, 0, stream>>>(g.nbins, bin_count, bin_start, sub_start, nsub,
-                                            maxsub);
-    JFA_HIP_CHECK(hipGetLastError());
+template <typename T>
+int launch_sort_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                       const SortScratch &s, int *bin_start, int *sub_start, int *nsub,
+                       int maxsub, int *idx, int *l0, T *tv, hipStream_t stream) {
+    if (M > 0) {
+        const unsigned blocks = (unsigned)((M + 255) / 256);
+        JFA_DISPATCH_DIM(g.dim, bin_keys_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
+                                    g, M, x, y, z, s.key_in, s.val_in));
+        JFA_HIP_CHECK(hipGetLastError());
+        size_t bytes = s.temp_bytes;
+        JFA_HIP_CHECK(rocprim::radix_sort_pairs(s.temp, bytes, s.key_in, s.key_out, s.val_in,
+                                                (unsigned *)idx, (unsigned)M, 0,
+                                                key_bits(g.nbins), stream));
+    }
+    {
+        const unsigned blocks = (unsigned)((g.nbins + 1 + 255) / 256);
+        bin_bounds_kernel<<<blocks, 256, 0, stream>>>(M, g.nbins, s.key_out, bin_start);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    if (sub_start) {
+        sub_scan_kernel<<<1, 1024, 0, stream>>>(g.nbins, bin_start, sub_start, nsub, maxsub);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    if (M > 0) {
+        const unsigned blocks = (unsigned)((M + 255) / 256);
+        JFA_DISPATCH_DIM(g.dim, gather_points_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
+                                    g, M, x, y, z, idx, l0, tv));
+        JFA_HIP_CHECK(hipGetLastError());
+    }
     return JFA_OK;
 }
 
 template <typename T>
-int launch_scatter(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
-                   const int *bin_of, const int *rank_of, const int *bin_start, int sorted,
-                   int *idx, int *l0, T *tv, hipStream_t stream) {
+int launch_fold_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                       int *idx, int *l0, T *tv, hipStream_t stream) {
     if (M == 0) return JFA_OK;
-    unsigned blocks = (unsigned)((M + 255) / 256);
-    JFA_DISPATCH_DIM(g.dim, scatter_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
-                                g, M, x, y, z, bin_of, rank_of, bin_start, sorted, idx, l0, tv));
+    const unsigned blocks = (unsigned)((M + 255) / 256);
+    JFA_DISPATCH_DIM(g.dim, fold_points_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
+                                g, M, x, y, z, idx, l0, tv));
     JFA_HIP_CHECK(hipGetLastError());
     return JFA_OK;
 }
 
-template int launch_fold_bin<float>(const GridGeom &, int64_t, const float *, const float *,
-                                    const float *, int *, int *, int *, hipStream_t);
-template int launch_fold_bin<double>(const GridGeom &, int64_t, const double *, const double *,
-                                     const double *, int *, int *, int *, hipStream_t);
-template int launch_scatter<float>(const GridGeom &, int64_t, const float *, const float *,
-                                   const float *, const int *, const int *, const int *, int,
-                                   int *, int *, float *, hipStream_t);
-template int launch_scatter<double>(const GridGeom &, int64_t, const double *, const double *,
-                                    const double *, const int *, const int *, const int *, int,
-                                    int *, int *, double *, hipStream_t);
+#define JFA_INST(T)                                                                          \
+    template int launch_sort_points<T>(const GridGeom &, int64_t, const T *, const T *,      \
+                                       const T *, const SortScratch &, int *, int *, int *,  \
+                                       int, int *, int *, T *, hipStream_t);                 \
+    template int launch_fold_points<T>(const GridGeom &, int64_t, const T *, const T *,      \
+                                       const T *, int *, int *, T *, hipStream_t);
+JFA_INST(float)
+JFA_INST(double)
+#undef JFA_INST
 
 }  // namespace jfa

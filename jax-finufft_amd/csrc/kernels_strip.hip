@@ -251,7 +251,7 @@ int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int 
                         int *piece_start, int *npieces, const T *coef, int kerevalmeth,
                         double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
                         T *scratch, int64_t scratch_stride, int ntr, hipStream_t stream) {
-    if (ntr == 0) return JFA_OK;
+    if (ntr == 0 || pts.M == 0) return JFA_OK;
     strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces);
     JFA_HIP_CHECK(hipGetLastError());
     const int64_t ub = sg.nunits + pts.M / sg.pmax + 1;

@@ -524,26 +524,31 @@ int jfa::default_maxsub(jfa_plan p) {
 int jfa::reserve_points(jfa_plan p, int64_t M) {
     const size_t real = p->is_double ? 8 : 4;
     if (p->geom.nbins + 1 > p->cap_bins) {
-        for (int **q : {&p->d_bin_count, &p->d_bin_start, &p->d_sub_start})
+        for (int **q : {&p->d_bin_start, &p->d_sub_start})
             free_ptr(reinterpret_cast<void *&>(*q));
         const size_t nb = (size_t)p->geom.nbins + 1;
-        // bin_count is padded to a multiple of 4 ints for the vector fill
-        JFA_HIP_CHECK(hipMalloc(&p->d_bin_count, sizeof(int) * (nb + 4)));
         JFA_HIP_CHECK(hipMalloc(&p->d_bin_start, sizeof(int) * nb));
         JFA_HIP_CHECK(hipMalloc(&p->d_sub_start, sizeof(int) * nb));
         if (!p->d_nsub) JFA_HIP_CHECK(hipMalloc(&p->d_nsub, sizeof(int) * 4));
         p->cap_bins = (int64_t)nb;
     }
     if (M > p->cap_M) {
-        for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_bin_of,
-                         (void **)&p->d_rank_of, &p->d_t})
+        for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_key_in,
+                         (void **)&p->d_val_in, (void **)&p->d_key_out, &p->d_t, &p->d_sort_temp})
             free_ptr(*q);
         const int64_t cap = std::max<int64_t>(M, 1);
         JFA_HIP_CHECK(hipMalloc(&p->d_idx, sizeof(int) * cap));
         JFA_HIP_CHECK(hipMalloc(&p->d_l0, sizeof(int) * cap * p->dim));
-        JFA_HIP_CHECK(hipMalloc(&p->d_bin_of, sizeof(int) * cap));
-        JFA_HIP_CHECK(hipMalloc(&p->d_rank_of, sizeof(int) * cap));
+        JFA_HIP_CHECK(hipMalloc(&p->d_key_in, sizeof(unsigned) * cap));
+        JFA_HIP_CHECK(hipMalloc(&p->d_val_in, sizeof(unsigned) * cap));
+        JFA_HIP_CHECK(hipMalloc(&p->d_key_out, sizeof(unsigned) * cap));
         JFA_HIP_CHECK(hipMalloc(&p->d_t, real * cap * p->dim));
+        p->sort_temp_bytes = sort_temp_bytes(cap);
+        if (!p->sort_temp_bytes) {
+            set_error("radix sort temp size query failed");
+            return JFA_ERR_HIP;
+        }
+        JFA_HIP_CHECK(hipMalloc(&p->d_sort_temp, p->sort_temp_bytes));
         p->cap_M = cap;
     }
     return JFA_OK;
@@ -552,21 +557,14 @@ int jfa::reserve_points(jfa_plan p, int64_t M) {
 template <typename T>
 static int sort_points(jfa_plan p, int64_t M, const T *x, const T *y, const T *z) {
     const GridGeom &g = p->geom;
-    const int sorted =
-        (p->method == JFA_METHOD_SM || p->method == JFA_METHOD_STRIP || p->opts.gpu_sort) ? 1 : 0;
-    int rc;
-    if (sorted) {
-        if ((rc = launch_fill_zero(p->d_bin_count, sizeof(int) * (g.nbins + 1), p->stream)))
-            return rc;
-        if ((rc = launch_fold_bin<T>(g, M, x, y, z, p->d_bin_of, p->d_rank_of, p->d_bin_count,
-                                     p->stream)))
-            return rc;
-        if ((rc = launch_bin_scan(g, p->d_bin_count, p->d_bin_start, p->d_sub_start, p->d_nsub,
-                                  p->maxsub, p->stream)))
-            return rc;
-    }
-    return launch_scatter<T>(g, M, x, y, z, p->d_bin_of, p->d_rank_of, p->d_bin_start, sorted,
-                             p->d_idx, p->d_l0, (T *)p->d_t, p->stream);
+    const bool sorted =
+        p->method == JFA_METHOD_SM || p->method == JFA_METHOD_STRIP || p->opts.gpu_sort;
+    if (!sorted)
+        return launch_fold_points<T>(g, M, x, y, z, p->d_idx, p->d_l0, (T *)p->d_t, p->stream);
+    SortScratch s{p->d_key_in, p->d_val_in, p->d_key_out, p->d_sort_temp, p->sort_temp_bytes};
+    return launch_sort_points<T>(g, M, x, y, z, s, p->d_bin_start,
+                                 p->method == JFA_METHOD_SM ? p->d_sub_start : nullptr, p->d_nsub,
+                                 p->maxsub, p->d_idx, p->d_l0, (T *)p->d_t, p->stream);
 }
 
 int jfa::setpts_12(jfa_plan p, int64_t M, const void *x, const void *y, const void *z) {
@@ -741,6 +739,68 @@ extern "C" int jfa_execute(jfa_plan p, void *c, void *fk) {
     return execute_12<float>(p, (float *)c, (float *)fk);
 }
 
+// Fused adjoint pass of a type-2 plan: val = the type-2 transform of fk at the
+// plan's points (optional), grad[d*M + j] = sum_t Re(conj(w_t,j) dval_t,j/dx_d)
+// (optional, needs w).  One amplify + FFT per batch, then one interpolation
+// pass that evaluates the kernel and its derivative together.
+template <typename T>
+static int interp_grad_t(jfa_plan p, const T *fk, const T *w, T *val, T *grad) {
+    const GridGeom &g = p->geom;
+    const SortedPts<T> pts = sorted_view<T>(p);
+    const BinIndex bi = bin_view(p);
+    const T *dk[3] = {(const T *)p->d_dk[0], (const T *)p->d_dk[1], (const T *)p->d_dk[2]};
+    const int64_t M = p->M;
+    T *fw = (T *)p->d_fw;
+    StageTimer tm(p);
+    for (int t0 = 0; t0 < p->ntransf; t0 += p->batch) {
+        const int nb = std::min(p->batch, p->ntransf - t0);
+        int rc = launch_amplify<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2], p->opts.modeord,
+                                   fk + 2 * (int64_t)t0 * p->Ntot, p->Ntot, fw, p->F, nb,
+                                   p->stream);
+        if (rc) return rc;
+        tm.mark("amplify");
+        if ((rc = fft_exec(p, fw, nb))) return rc;
+        tm.mark("fft");
+        rc = launch_interp_grad<T>(g, pts, bi, (const T *)p->d_coef, fw, p->F, nb,
+                                   w ? w + 2 * (int64_t)t0 * M : nullptr, M,
+                                   val ? val + 2 * (int64_t)t0 * M : nullptr, M, grad, M, t0 > 0,
+                                   p->stream);
+        if (rc) return rc;
+        tm.mark("interp_grad");
+    }
+    tm.report("interp_grad");
+    return JFA_OK;
+}
+
+extern "C" int jfa_interp_grad(jfa_plan p, const void *fk, const void *w, void *val,
+                               void *grad) {
+    if (!p) {
+        set_error("plan is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    if (p->type != 2 || p->opts.gpu_spreadinterponly) {
+        set_error("interp_grad needs a type-2 plan");
+        return JFA_ERR_BAD_ARG;
+    }
+    if (p->method != JFA_METHOD_SM) {
+        set_error("interp_grad needs the LDS-tile method (gpu_method 0 or 2)");
+        return JFA_ERR_NOT_SUPPORTED;
+    }
+    if ((grad && !w) || (p->M > 0 && !fk)) {
+        set_error("interp_grad: weights are required for the gradient");
+        return JFA_ERR_BAD_ARG;
+    }
+    DeviceGuard dg;
+    if (int rc = dg.set(p->device)) return rc;
+    p->used = true;
+    if (p->M == 0) return JFA_OK;
+    if (p->is_double)
+        return interp_grad_t<double>(p, (const double *)fk, (const double *)w, (double *)val,
+                                     (double *)grad);
+    return interp_grad_t<float>(p, (const float *)fk, (const float *)w, (float *)val,
+                                (float *)grad);
+}
+
 extern "C" int jfa_set_stream(jfa_plan p, jfa_stream_t stream) {
     if (!p) {
         set_error("plan is NULL");
@@ -773,8 +833,8 @@ extern "C" int jfa_destroy(jfa_plan p) {
     free_ptr(p->d_coef);
     for (auto &d : p->d_dk) free_ptr(d);
     free_ptr(p->d_fw);
-    for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_bin_of,
-                     (void **)&p->d_rank_of, &p->d_t, (void **)&p->d_bin_count,
+    for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_key_in,
+                     (void **)&p->d_val_in, (void **)&p->d_key_out, &p->d_sort_temp, &p->d_t,
                      (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub,
                      (void **)&p->d_piece_start, (void **)&p->d_npieces, &p->d_strip_scratch})
         free_ptr(*q);

@@ -45,9 +45,12 @@ struct jfa_plan_s {
 
     // sorted points
     int64_t M = 0, cap_M = 0, cap_bins = 0;
-    int *d_idx = nullptr, *d_l0 = nullptr, *d_bin_of = nullptr, *d_rank_of = nullptr;
+    int *d_idx = nullptr, *d_l0 = nullptr;
     void *d_t = nullptr;
-    int *d_bin_count = nullptr, *d_bin_start = nullptr, *d_sub_start = nullptr, *d_nsub = nullptr;
+    unsigned *d_key_in = nullptr, *d_val_in = nullptr, *d_key_out = nullptr;  // radix sort
+    void *d_sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    int *d_bin_start = nullptr, *d_sub_start = nullptr, *d_nsub = nullptr;
 
     // 2D strip spreader
     jfa::StripGeom sg{};

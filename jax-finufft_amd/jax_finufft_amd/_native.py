@@ -58,6 +58,7 @@ class JfaOpts(ctypes.Structure):
 ENTRY_POINTS = [
     f"jfa_nufft{d}d{t}{s}" for t in (1, 2, 3) for d in (1, 2, 3) for s in ("f", "")
 ]
+GRAD_ENTRY_POINTS = [f"jfa_nufft{d}d2{s}_grad" for d in (1, 2, 3) for s in ("f", "")]
 PLAN_API = [
     "jfa_default_opts", "jfa_makeplan", "jfa_setpts", "jfa_execute", "jfa_set_stream",
     "jfa_destroy", "jfa_plan_info", "jfa_last_error", "jfa_cache_size", "jfa_cache_clear",
@@ -114,6 +115,19 @@ def lib():
                 + [c_void_p] * nptr
             )
             fn.restype = c_int
+        for name in GRAD_ENTRY_POINTS:
+            fn = getattr(L, name)
+            single = name.endswith("f_grad")
+            d = int(name[len("jfa_nufft")])
+            fn.argtypes = (
+                [c_void_p, c_float if single else c_double]
+                + [c_int64] * 8
+                + [c_double]
+                + [c_int64] * 5
+                + [c_void_p] * (1 + d + 3)
+            )
+            fn.restype = c_int
+        L.jfa_interp_grad.argtypes = [c_void_p] * 5
         _lib = L
         return _lib
 
@@ -201,6 +215,51 @@ def lowering(source, *points, output_shape, iflag, eps, opts, nufft_type):
         )
     check(rc, f"nufft{ndim}d{nufft_type}{suffix}")
     return out
+
+
+def interp_grad(modes, points, weights, *, iflag, eps, opts, want_value):
+    """Fused type-2 adjoint pass through ``jfa_nufft{d}d2{f}_grad``.
+
+    ``modes`` is (n_tot, n_transf, *N), ``points`` are (n_tot, M) in user
+    order, ``weights`` (n_tot, n_transf, M) or None.  Returns
+    ``(value, grads)``: value = nufft2(modes, points, iflag) in canonical layout
+    (or None), and for each user dimension d the real (n_tot, M) array
+    ``sum_t Re(conj(weights_t) d value_t / d x_d)`` (empty list without weights).
+    Raises ``NotImplementedError`` when the plan cannot use the fused kernel.
+    """
+    ndim = len(points)
+    single = modes.dtype == torch.complex64
+    _require_device(modes, *points)
+    n_tot, n_transf = modes.shape[0], modes.shape[1]
+    n_j = points[0].shape[1]
+    n_k = list(modes.shape[2:])[::-1] + [0] * (3 - ndim)
+    if opts is None:
+        opts = options.Opts()
+    opts = options.unpack_opts(opts, 2, True)
+    native = opts.to_cufinufft_opts()
+    modes = modes.contiguous()
+    pts = [p.contiguous() for p in points[::-1]]
+    val = torch.empty((n_tot, n_transf, n_j), dtype=modes.dtype, device=modes.device) \
+        if want_value else None
+    grad = None
+    if weights is not None:
+        weights = weights.contiguous()
+        grad = torch.empty((n_tot, ndim, n_j), dtype=pts[0].dtype, device=modes.device)
+    fn = getattr(lib(), f"jfa_nufft{ndim}d2{'f' if single else ''}_grad")
+    ptr = lambda t: c_void_p(t.data_ptr()) if t is not None else c_void_p(None)  # noqa: E731
+    with torch.cuda.device(modes.device):
+        rc = fn(
+            _stream_handle(modes.device), float(eps), int(iflag), int(n_tot), int(n_transf),
+            int(n_j), int(n_k[0]), int(n_k[1]), int(n_k[2]), int(native.modeord),
+            float(native.upsampfac), int(native.gpu_method), int(native.gpu_sort),
+            int(native.gpu_kerevalmeth), int(native.gpu_maxbatchsize), int(native.debug),
+            ptr(modes), *[ptr(p) for p in pts], ptr(weights), ptr(val), ptr(grad),
+        )
+    if rc == 6:  # JFA_ERR_NOT_SUPPORTED: the plan's method has no fused variant
+        raise NotImplementedError(last_error())
+    check(rc, f"nufft{ndim}d2 fused adjoint")
+    grads = [] if grad is None else [grad[:, ndim - 1 - d] for d in range(ndim)]
+    return val, grads
 
 
 class Plan:

@@ -189,7 +189,7 @@ def _adjoint(nufft_type, grad, source, points, output_shape, iflag, eps, opts, n
     need_src, need_pts = need[0], need[1:]
     want_pts = any(need_pts)
     fopts = options.unpack_opts(opts, nufft_type, True)
-    fused = want_pts and bool(getattr(fopts, "gpu_fused_adjoint", True)) and _fused_available()
+    fused = want_pts and bool(getattr(fopts, "gpu_fused_adjoint", True))
     modeord = _modeord(fopts)
     out = [None] * (1 + ndim)
 
@@ -197,15 +197,20 @@ def _adjoint(nufft_type, grad, source, points, output_shape, iflag, eps, opts, n
         # grad_c = nufft2(G, x, -iflag); point grads from d/dx of the same evaluation
         bopts = options.unpack_opts(opts, 2, False)
         n_modes = tuple(output_shape)
-        if want_pts and fused:
-            val, dval = _fused_interp_grad(grad, points, -iflag, eps, bopts)
-            if need_src:
+        if fused:
+            try:
+                # grad_x_d = sum_t Re(c * conj(h_d)) = sum_t Re(conj(c) h_d), h_d the
+                # x_d-derivative of nufft2(G, ., -iflag): weights are the strengths
+                val, grads = _native.interp_grad(grad, points, source, iflag=-iflag, eps=eps,
+                                                 opts=bopts, want_value=need_src)
+            except NotImplementedError:
+                fused = False
+            else:
                 out[0] = val
-            for d in range(ndim):
-                if need_pts[d]:
-                    # dval[d] is d/dx of nufft2(G, ., -iflag) at the points
-                    out[1 + d] = (source * dval[d].conj()).real.sum(dim=1)
-            return out
+                for d in range(ndim):
+                    if need_pts[d]:
+                        out[1 + d] = grads[d]
+                return out
         args = [grad]
         if want_pts:
             for d in range(ndim):
@@ -226,16 +231,24 @@ def _adjoint(nufft_type, grad, source, points, output_shape, iflag, eps, opts, n
         out[0] = _bind(1, grad, points, n_modes, -iflag, eps, bopts)
     if want_pts:
         if fused:
-            _, dval = _fused_interp_grad(source, points, iflag, eps, fopts)
-            derivs = dval
-        else:
+            try:
+                # grad_x_d = sum_t Re(conj(g) dc/dx_d): weights are the cotangents
+                _, grads = _native.interp_grad(source, points, grad, iflag=iflag, eps=eps,
+                                               opts=fopts, want_value=False)
+            except NotImplementedError:
+                fused = False
+            else:
+                for d in range(ndim):
+                    if need_pts[d]:
+                        out[1 + d] = grads[d]
+        if not fused:
             args = [source * _freq_weights(n_modes, d, modeord, iflag, source)
                     for d in range(ndim)]
             res = _bind(2, _stack(args).contiguous(), points, None, iflag, eps, fopts)
             derivs = _unstack(res, ndim)
-        for d in range(ndim):
-            if need_pts[d]:
-                out[1 + d] = (grad.conj() * derivs[d]).real.sum(dim=1)
+            for d in range(ndim):
+                if need_pts[d]:
+                    out[1 + d] = (grad.conj() * derivs[d]).real.sum(dim=1)
     return out
 
 
@@ -269,14 +282,6 @@ def _adjoint3(grad, source, points, iflag, eps, opts, need):
             if need_s[d]:
                 out[1 + ndim + d] = (grad.conj() * parts[d]).real.sum(dim=1)
     return out
-
-
-def _fused_available():
-    return hasattr(_native, "interp_grad")
-
-
-def _fused_interp_grad(modes, points, iflag, eps, opts):
-    return _native.interp_grad(modes, points, iflag=iflag, eps=eps, opts=opts)
 
 
 # ------------------------------------------------------------------ vmap

@@ -1,0 +1,248 @@
+"""GPU tests of differentiation and batching.
+
+* torch.autograd.gradcheck (finite differences of the whole op) in fp64, the
+  counterpart of the reference's jtu.check_grads (tests/ops_test.py:128-219,
+  :545, :567);
+* the fused adjoint (one transform + one gradient-interpolation pass) against
+  the reference's stacked multi-NUFFT rule, and both against closed forms
+  evaluated by the FP64 direct-sum oracle;
+* FP64 central differences of loss = sum |nufft2(f, x, y) - d|^2 (the
+  differentiation workload, at a reduced size);
+* vmap through torch.func (the reference's batch rule, ops_test.py:222-377).
+"""
+
+from itertools import product
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gpu_helpers import cnormal, dev, host, rel_l2
+from jax_finufft_amd import nufft1, nufft2, nufft3
+from jax_finufft_amd.options import Opts
+
+pytestmark = pytest.mark.gpu
+
+FUSED = Opts(gpu_fused_adjoint=True)
+PLAIN = Opts(gpu_fused_adjoint=False)
+
+
+def _modes(ndim, n=35):
+    return tuple(int(v) for v in (n // ndim + 5 * np.arange(ndim)))
+
+
+# ------------------------------------------------------------ gradcheck
+
+
+@pytest.mark.parametrize("ndim,iflag,opts", product([1, 2, 3], [-1, 1], [FUSED, PLAIN]))
+def test_nufft1_gradcheck(ndim, iflag, opts):
+    rng = np.random.default_rng(657)
+    M = 12
+    n_modes = _modes(ndim, 12)
+    x = [dev(rng.uniform(-np.pi, np.pi, M)).requires_grad_() for _ in range(ndim)]
+    c = dev(cnormal(rng, M, "f64")).requires_grad_()
+    fn = lambda c, *x: nufft1(n_modes, c, *x, eps=1e-12, iflag=iflag, opts=opts)  # noqa: E731
+    assert torch.autograd.gradcheck(fn, (c, *x), eps=1e-6, atol=1e-5, rtol=1e-4)
+
+
+@pytest.mark.parametrize("ndim,iflag,opts", product([1, 2, 3], [-1, 1], [FUSED, PLAIN]))
+def test_nufft2_gradcheck(ndim, iflag, opts):
+    rng = np.random.default_rng(657)
+    M = 12
+    n_modes = _modes(ndim, 12)
+    x = [dev(rng.uniform(-np.pi, np.pi, M)).requires_grad_() for _ in range(ndim)]
+    f = dev(cnormal(rng, n_modes, "f64")).requires_grad_()
+    fn = lambda f, *x: nufft2(f, *x, eps=1e-12, iflag=iflag, opts=opts)  # noqa: E731
+    assert torch.autograd.gradcheck(fn, (f, *x), eps=1e-6, atol=1e-5, rtol=1e-4)
+
+
+@pytest.mark.parametrize("ndim,iflag", product([1, 2, 3], [-1, 1]))
+def test_nufft3_gradcheck(ndim, iflag):
+    rng = np.random.default_rng(657)
+    x = [dev(rng.uniform(-1, 1, 10)).requires_grad_() for _ in range(ndim)]
+    s = [dev(rng.uniform(-1, 1, 8)).requires_grad_() for _ in range(ndim)]
+    c = dev(cnormal(rng, 10, "f64")).requires_grad_()
+    fn = lambda c, *p: nufft3(c, *p, eps=1e-12, iflag=iflag)  # noqa: E731
+    # the type-3 adjoint spreads with float atomics, so repeated backward passes
+    # agree to rounding, not bit for bit
+    assert torch.autograd.gradcheck(fn, (c, *x, *s), eps=1e-6, atol=1e-5, rtol=1e-4,
+                                    nondet_tol=1e-12)
+
+
+# ------------------------------------------------------------ closed forms
+
+
+def _kd(n, modeord=0):
+    return np.arange(-(n // 2), (n + 1) // 2) if modeord == 0 else np.fft.fftfreq(n, 1 / n)
+
+
+@pytest.mark.parametrize("ndim,iflag", product([1, 2, 3], [-1, 1]))
+def test_fused_type2_point_gradient_closed_form(ndim, iflag):
+    # d/dx_d of c_j = sum_k f_k e^{i s k.x_j} is the type 2 of (i s k_d f_k):
+    # grad_x_d = Re(conj(g_j) * type2(i s k_d f)_j), evaluated with the oracle
+    rng = np.random.default_rng(21)
+    M = 400
+    n_modes = _modes(ndim, 40)
+    x = [rng.uniform(-np.pi, np.pi, M) for _ in range(ndim)]
+    f = cnormal(rng, n_modes, "f64")
+    g = cnormal(rng, M, "f64")
+    xt = [dev(a).requires_grad_() for a in x]
+    out = nufft2(dev(f), *xt, eps=1e-12, iflag=iflag, opts=FUSED)
+    out.backward(dev(g))
+    for d in range(ndim):
+        shape = [1] * ndim
+        shape[d] = -1
+        k = _kd(n_modes[d]).reshape(shape)
+        deriv = oracle.type2(1j * iflag * k * f, x, iflag)
+        expect = np.real(np.conj(g) * deriv)
+        assert rel_l2(host(xt[d].grad), expect) < 1e-9
+
+
+@pytest.mark.parametrize("ndim,iflag", product([1, 2, 3], [-1, 1]))
+def test_fused_type1_gradients_closed_form(ndim, iflag):
+    # f = A c:  grad_c = type2(G, -s);  grad_x_d = Re(c conj(type2(-i s k_d G, -s)))
+    rng = np.random.default_rng(22)
+    M = 300
+    n_modes = _modes(ndim, 40)
+    x = [rng.uniform(-np.pi, np.pi, M) for _ in range(ndim)]
+    c = cnormal(rng, M, "f64")
+    G = cnormal(rng, n_modes, "f64")
+    xt = [dev(a).requires_grad_() for a in x]
+    ct = dev(c).requires_grad_()
+    nufft1(n_modes, ct, *xt, eps=1e-12, iflag=iflag, opts=FUSED).backward(dev(G))
+    assert rel_l2(host(ct.grad), oracle.type2(G, x, -iflag)) < 1e-10
+    for d in range(ndim):
+        shape = [1] * ndim
+        shape[d] = -1
+        k = _kd(n_modes[d]).reshape(shape)
+        h = oracle.type2(-1j * iflag * k * G, x, -iflag)
+        assert rel_l2(host(xt[d].grad), np.real(c * np.conj(h))) < 1e-9
+
+
+@pytest.mark.parametrize("nufft_type,ndim,prec", product([1, 2], [1, 2, 3], ["f32", "f64"]))
+def test_fused_matches_plain_rule(nufft_type, ndim, prec):
+    rng = np.random.default_rng(23)
+    M, T = 2000, 3
+    n_modes = _modes(ndim, 48)
+    rdt = np.float32 if prec == "f32" else np.float64
+    eps = 1e-6 if prec == "f32" else 1e-11
+    x = [rng.uniform(-np.pi, np.pi, M).astype(rdt) for _ in range(ndim)]
+    grads = {}
+    for name, opts in (("fused", FUSED), ("plain", PLAIN)):
+        xt = [dev(a).requires_grad_() for a in x]
+        if nufft_type == 1:
+            src = dev(cnormal(np.random.default_rng(5), (T, M), prec)).requires_grad_()
+            out = nufft1(n_modes, src, *xt, eps=eps, opts=opts)
+        else:
+            src = dev(cnormal(np.random.default_rng(5), (T,) + n_modes, prec)).requires_grad_()
+            out = nufft2(src, *xt, eps=eps, opts=opts)
+        cot = dev(cnormal(np.random.default_rng(6), tuple(out.shape), prec))
+        out.backward(cot)
+        grads[name] = [host(src.grad)] + [host(a.grad) for a in xt]
+    tol = 2e-4 if prec == "f32" else 1e-9
+    for a, b in zip(grads["fused"], grads["plain"]):
+        assert rel_l2(a, b) < tol
+
+
+def test_loss_gradient_central_differences():
+    # loss = sum |nufft2(f, x, y) - d|^2 in 2D, FP64, point gradient vs central
+    # differences (h = 1e-6, eps = 1e-12) on sampled coordinates
+    rng = np.random.default_rng(6)
+    n = 64
+    M = 4000
+    k0 = np.arange(-(n // 2), n // 2)
+    f = np.exp(-(k0[:, None] ** 2 + k0[None, :] ** 2) / (2 * 8.0 ** 2)) * cnormal(rng, (n, n), "f64")
+    h = 2 * np.pi / 64
+    base = rng.uniform(-np.pi, np.pi, (2, M))
+    x, y = np.mod(base + rng.uniform(-h / 2, h / 2, (2, M)) + np.pi, 2 * np.pi) - np.pi
+    d = cnormal(rng, M, "f64")
+    ft = dev(f)
+
+    def loss(xa, ya):
+        out = nufft2(ft, xa, ya, eps=1e-12)
+        return torch.sum(torch.abs(out - dev(d)) ** 2)
+
+    xt, yt = dev(x).requires_grad_(), dev(y).requires_grad_()
+    loss(xt, yt).backward()
+    gx, gy = host(xt.grad), host(yt.grad)
+    ftg = dev(f).requires_grad_()
+    out = nufft2(ftg, dev(x), dev(y), eps=1e-12)
+    torch.sum(torch.abs(out - dev(d)) ** 2).backward()
+    gf = host(ftg.grad)
+    step = 1e-6
+    for j in rng.choice(M, 16, replace=False):
+        for arr, g in ((x, gx), (y, gy)):
+            ap, am = arr.copy(), arr.copy()
+            ap[j] += step
+            am[j] -= step
+            xs_p = (ap, y) if arr is x else (x, ap)
+            xs_m = (am, y) if arr is x else (x, am)
+            lp = float(loss(dev(xs_p[0]), dev(xs_p[1])))
+            lm = float(loss(dev(xs_m[0]), dev(xs_m[1])))
+            fd = (lp - lm) / (2 * step)
+            assert abs(fd - g[j]) <= 1e-5 * max(1.0, abs(g[j]))
+    for (a, b) in [tuple(rng.integers(0, n, 2)) for _ in range(16)]:
+        for part in (1.0, 1j):
+            fp, fm = f.copy(), f.copy()
+            fp[a, b] += part * step
+            fm[a, b] -= part * step
+            lp = float(torch.sum(torch.abs(nufft2(dev(fp), dev(x), dev(y), eps=1e-12) - dev(d)) ** 2))
+            lm = float(torch.sum(torch.abs(nufft2(dev(fm), dev(x), dev(y), eps=1e-12) - dev(d)) ** 2))
+            fd = (lp - lm) / (2 * step)
+            # torch's complex gradient is dL/dRe + i dL/dIm
+            g = gf[a, b].real if part == 1.0 else gf[a, b].imag
+            assert abs(fd - g) <= 1e-5 * max(1.0, abs(g))
+
+
+# ------------------------------------------------------------ vmap
+
+
+@pytest.mark.parametrize("ndim,iflag", product([1, 2, 3], [-1, 1]))
+def test_nufft1_vmap(ndim, iflag):
+    rng = np.random.default_rng(657)
+    R, M = 5, 50
+    n_modes = _modes(ndim)
+    x = [dev(rng.uniform(-np.pi, np.pi, (R, M))) for _ in range(ndim)]
+    c = dev(cnormal(rng, (R, M), "f64"))
+    func = lambda c, *x: nufft1(n_modes, c, *x, iflag=iflag)  # noqa: E731
+    calc = torch.func.vmap(func)(c, *x)
+    for n in range(R):
+        assert rel_l2(host(calc[n]), host(func(c[n], *(a[n] for a in x)))) < 1e-12
+    calc_ax = torch.func.vmap(func, in_dims=(1,) + (0,) * ndim)(c.T.contiguous(), *x)
+    assert rel_l2(host(calc_ax), host(calc)) < 1e-12
+    calc_unmap = torch.func.vmap(func, in_dims=(None,) + (0,) * ndim)(c[0], *x)
+    for n in range(R):
+        assert rel_l2(host(calc_unmap[n]), host(func(c[0], *(a[n] for a in x)))) < 1e-12
+    calc_pt = torch.func.vmap(func, in_dims=(0,) + (0,) * (ndim - 1) + (None,))(
+        c, *x[:-1], x[-1][0])
+    for n in range(R):
+        ref = func(c[n], *(a[n] for a in x[:-1]), x[-1][0])
+        assert rel_l2(host(calc_pt[n]), host(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_nufft2_and_nufft3_vmap(ndim):
+    rng = np.random.default_rng(657)
+    R, M = 4, 40
+    n_modes = _modes(ndim)
+    x = [dev(rng.uniform(-np.pi, np.pi, (R, M))) for _ in range(ndim)]
+    s = [dev(rng.uniform(-1, 1, (R, 30))) for _ in range(ndim)]
+    f = dev(cnormal(rng, (R,) + n_modes, "f64"))
+    c = dev(cnormal(rng, (R, M), "f64"))
+    calc = torch.func.vmap(nufft2)(f, *x)
+    calc3 = torch.func.vmap(nufft3)(c, *x, *s)
+    for n in range(R):
+        assert rel_l2(host(calc[n]), host(nufft2(f[n], *(a[n] for a in x)))) < 1e-12
+        ref3 = nufft3(c[n], *(a[n] for a in x), *(a[n] for a in s))
+        assert rel_l2(host(calc3[n]), host(ref3)) < 1e-12
+
+
+def test_gradients_have_input_shapes():
+    # the reference's gh54 regressions: shapes of jvp/grad through a batched op
+    rng = np.random.default_rng(54)
+    f = dev(cnormal(rng, (8, 1000), "f64")).requires_grad_()
+    x = dev(rng.normal(size=(8, 3, 1000))).requires_grad_()
+    out = nufft1((16, 16, 16), f, x[:, 0], x[:, 1], x[:, 2], iflag=-1)
+    torch.real(out).mean().backward()
+    assert f.grad.shape == f.shape and x.grad.shape == x.shape
