@@ -75,6 +75,7 @@ struct GridGeom {
     int ns;              // kernel width
     int dim;
     int ymajor;          // 2D bins ordered y fastest (bin = by + nb1*bx), for strips
+    int keyshift;        // sort key = bin << keyshift | ((l0x mod bs0) >> 3); 0 = bin only
 };
 
 // Geometry of the 2D strip spreader (kernels_strip.hip): strips of SW = 64 -
@@ -135,7 +136,10 @@ template <typename T>
 int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, const T *coef, int kerevalmeth,
                         double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
-                        T *scratch, int64_t scratch_stride, int ntr, hipStream_t stream);
+                        T *scratch, int64_t scratch_stride, int ntr, int phases,
+                        hipStream_t stream);
+// phases of launch_spread_strip (a bit mask): piece table, spreading, merge
+constexpr int kStripUnits = 1, kStripSpread = 2, kStripMerge = 4, kStripAll = 7;
 template <typename T>
 int launch_interp(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
                   const T *coef, int kerevalmeth, double beta, const T *fw, int64_t fw_stride,

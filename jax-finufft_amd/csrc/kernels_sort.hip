@@ -21,11 +21,14 @@ namespace jfa {
 
 namespace {
 
+// Sort key of point j: its bin, and with keyshift = 3 (the 2D strip spreader)
+// also the 8-column block of its first tap within the strip as the low bits.
 template <typename T, int DIM>
-__device__ __forceinline__ unsigned bin_of_point(const GridGeom &g, const T *x, const T *y,
+__device__ __forceinline__ unsigned key_of_point(const GridGeom &g, const T *x, const T *y,
                                                  const T *z, int64_t j) {
     Folded f0 = fold_point((double)x[j], g.nf[0], g.scale[0], g.shift[0], g.ns);
     int64_t b = f0.l0w / g.bs[0];
+    const int sub = (f0.l0w - (int)b * g.bs[0]) >> 3;
     if constexpr (DIM > 1) {
         Folded f1 = fold_point((double)y[j], g.nf[1], g.scale[1], g.shift[1], g.ns);
         if (DIM == 2 && g.ymajor)
@@ -37,7 +40,7 @@ __device__ __forceinline__ unsigned bin_of_point(const GridGeom &g, const T *x, 
         Folded f2 = fold_point((double)z[j], g.nf[2], g.scale[2], g.shift[2], g.ns);
         b += (int64_t)(f2.l0w / g.bs[2]) * g.nb[0] * g.nb[1];
     }
-    return (unsigned)b;
+    return g.keyshift ? (unsigned)((b << g.keyshift) | sub) : (unsigned)b;
 }
 
 template <typename T, int DIM>
@@ -46,20 +49,21 @@ __global__ void __launch_bounds__(256)
                     const T *__restrict__ z, unsigned *key, unsigned *val) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= M) return;
-    key[j] = bin_of_point<T, DIM>(g, x, y, z, j);
+    key[j] = key_of_point<T, DIM>(g, x, y, z, j);
     val[j] = (unsigned)j;
 }
 
-// bin_start[b] = first p with skey[p] >= b, for b in [0, nbins]
+// bin_start[b] = first p with skey[p] >= b << shift, for b in [0, nbins]
 __global__ void __launch_bounds__(256)
-    bin_bounds_kernel(int64_t M, int64_t nbins, const unsigned *__restrict__ skey,
+    bin_bounds_kernel(int64_t M, int64_t nbins, int shift, const unsigned *__restrict__ skey,
                       int *bin_start) {
     const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (b > nbins) return;
+    const int64_t kb = b << shift;
     int64_t lo = 0, hi = M;  // answer in [lo, hi]
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)skey[mid] < b)
+        if ((int64_t)skey[mid] < kb)
             lo = mid + 1;
         else
             hi = mid;
@@ -193,11 +197,12 @@ int launch_sort_points(const GridGeom &g, int64_t M, const T *x, const T *y, con
         size_t bytes = s.temp_bytes;
         JFA_HIP_CHECK(rocprim::radix_sort_pairs(s.temp, bytes, s.key_in, s.key_out, s.val_in,
                                                 (unsigned *)idx, (unsigned)M, 0,
-                                                key_bits(g.nbins), stream));
+                                                key_bits(g.nbins) + g.keyshift, stream));
     }
     {
         const unsigned blocks = (unsigned)((g.nbins + 1 + 255) / 256);
-        bin_bounds_kernel<<<blocks, 256, 0, stream>>>(M, g.nbins, s.key_out, bin_start);
+        bin_bounds_kernel<<<blocks, 256, 0, stream>>>(M, g.nbins, g.keyshift, s.key_out,
+                                                      bin_start);
         JFA_HIP_CHECK(hipGetLastError());
     }
     if (sub_start) {

@@ -9,11 +9,15 @@
 // Layout.  The fine grid is cut into vertical strips of SW = 64 - ns + 1
 // columns; strip s owns points whose first x-tap l0x lies in
 // [s*SW, (s+1)*SW), so all of their taps fall in the 64 columns
-// [s*SW, s*SW + 64).  Points are counting-sorted by (strip, l0y) (the
-// "y-major" bins set up by the plan), so within a strip they arrive in row
-// order.  A strip is cut into units of H consecutive l0y rows.
+// [s*SW, s*SW + 64).  Points are sorted by (strip, l0y) (the "y-major" bins
+// set up by the plan; for ns <= 8 also by 8-column block within the row), so
+// within a strip they arrive in row order.  A strip is cut into units of H
+// consecutive l0y rows.
 //
-// One wave per unit: lane x owns column s*SW + x and holds a rolling window of
+// For ns <= 8, spread_strip8_kernel below assigns cells to lanes modulo 8 in
+// both directions, so that every lane does useful work for every point.  For
+// wider kernels, one wave per unit: lane x owns column s*SW + x and holds a
+// rolling window of
 // ns rows (real and imaginary parts) in registers.  Points are staged 64 at a
 // time: each lane evaluates one point's ns x-weights and ns y-weights (Horner)
 // and writes the x-weights and c*y-weights to wave-private LDS; then the wave
@@ -159,6 +163,171 @@ __global__ void __launch_bounds__(kStripLanes)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Modular-ownership variant for ns <= 8 (the fp32 widths).  The unit, its
+// pieces, its scratch tile and the merge are those of spread_strip_kernel; what
+// changes is which lane holds which cell.  Lane (a, r) = (lane & 7, lane >> 3)
+// holds the 8 cells of one row whose columns are a, a+8, ..., a+56 of the strip
+// (accumulator k is column 8k + a), for the window row congruent to r mod 8.
+// An ns x ns footprint then puts exactly one cell in every lane (ns = 8) and
+// each lane adds one complex product per point instead of ns.
+//
+// Which of its eight accumulators a lane adds to depends on the point's 8-column
+// block bx = X >> 3 (X = first tap within the strip) and on whether the lane's
+// column a lies before the point's phase X & 7 (then it is the next block).
+// The first is made wave-uniform by sorting the points by (row, bx) (sort key
+// low bits) and walking them in runs of equal (row, bx) under a switch; the
+// second by staging, per point and lane column, the weight split into a pair
+// (weight in block bx, weight in block bx + 1), one of which is zero.
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair {
+    T x, y;
+};
+
+template <typename T, int BX>
+__device__ __forceinline__ void strip8_walk(const Pair<T> *__restrict__ s_w,
+                                            const Pair<T> *__restrict__ s_cky, int q0, int q1,
+                                            int a, int dy, T (&acc_r)[8], T (&acc_i)[8]) {
+#pragma unroll 4
+    for (int q = q0; q < q1; ++q) {
+        const Pair<T> w = s_w[q * 8 + a];
+        const Pair<T> cy = s_cky[q * 8 + dy];
+        acc_r[BX] = fma(w.x, cy.x, acc_r[BX]);
+        acc_i[BX] = fma(w.x, cy.y, acc_i[BX]);
+        if constexpr (BX < 7) {  // block 8 would only receive zero weights
+            acc_r[BX + 1] = fma(w.y, cy.x, acc_r[BX + 1]);
+            acc_i[BX + 1] = fma(w.y, cy.y, acc_i[BX + 1]);
+        }
+    }
+}
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(kStripLanes)
+    spread_strip8_kernel(StripGeom sg, SortedPts<T> pts, const int *__restrict__ bin_start,
+                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
+                         const T *__restrict__ coef, T beta, int horner, const T *__restrict__ c,
+                         int64_t c_stride, T *__restrict__ fw, int64_t fw_stride,
+                         T *__restrict__ scratch, int64_t scratch_stride) {
+    static_assert(NS <= 8, "modular ownership needs ns <= 8");
+    // per staged point q: s_w[q*8 + a] = weight pair of lane column a,
+    // s_cky[q*8 + dy] = c * (y-weight of row offset dy), zero for dy >= NS
+    __shared__ Pair<T> s_w[kStripLanes * 8];
+    __shared__ Pair<T> s_cky[kStripLanes * 8];
+    c += 2 * c_stride * blockIdx.y;
+    fw += 2 * fw_stride * blockIdx.y;
+    scratch += 2 * scratch_stride * blockIdx.y;
+    const int lane = threadIdx.x;
+    const int a = lane & 7, r = lane >> 3;
+    const int total = *npieces;
+    const int rows_tile = sg.H + NS - 1;
+    for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
+        const int u = piece_unit(piece_start, sg.nunits, piece);
+        const int sub = piece - piece_start[u];
+        const bool heavy = piece_start[u + 1] - piece_start[u] > 1;
+        const int s = u / sg.K, k = u - s * sg.K;
+        const int ox = s * sg.SW;
+        const int r0 = k * sg.H;
+        const int r1 = min(r0 + sg.H, (int)sg.nf1);
+        const int ub = bin_start[(int64_t)s * sg.nf1 + r0];
+        const int ue = bin_start[(int64_t)s * sg.nf1 + r1];
+        const int pb = heavy ? ub + sub * sg.pmax : ub;
+        const int pe = heavy ? min(pb + sg.pmax, ue) : ue;
+        T acc_r[8], acc_i[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
+        // the window holds rows R .. R+7; row y sits in the lanes with r == y & 7
+        int R = heavy ? (pe > pb ? pts.l0[1][pb] : r0) : r0;
+        T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
+        auto flush = [&](int row) {
+            if ((row & 7) == r) {
+                if (heavy) {
+                    const int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        int64_t x = ox + 8 * b + a;
+                        if (x >= sg.nf0) x -= sg.nf0;
+                        const int64_t gi = y * sg.nf0 + x;
+                        if (acc_r[b] != T(0) || acc_i[b] != T(0)) {
+                            atomic_add_global(&fw[2 * gi], acc_r[b]);
+                            atomic_add_global(&fw[2 * gi + 1], acc_i[b]);
+                        }
+                    }
+                } else {
+                    Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile) +
+                                   (int64_t)(row - r0) * kStripLanes + a;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) dst[8 * b] = Pair<T>{acc_r[b], acc_i[b]};
+                }
+#pragma unroll
+                for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
+            }
+        };
+        for (int bb = pb; bb < pe; bb += kStripLanes) {
+            const int nb = min(kStripLanes, pe - bb);
+            int key = -1;  // (row << 3) | block of the lane's staged point
+            if (lane < nb) {
+                const int p = bb + lane;
+                const int64_t j = pts.idx[p];
+                const T cr = c[2 * j], ci = c[2 * j + 1];
+                T kx[NS], ky[NS];
+                if (horner) {
+                    eval_kernel<T, NS, true>(coef, beta, pts.t[0][p], kx);
+                    eval_kernel<T, NS, true>(coef, beta, pts.t[1][p], ky);
+                } else {
+                    eval_kernel<T, NS, false>(coef, beta, pts.t[0][p], kx);
+                    eval_kernel<T, NS, false>(coef, beta, pts.t[1][p], ky);
+                }
+                const int X = pts.l0[0][p] - ox;
+                const int ph = X & 7;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const T v = i < NS ? kx[i] : T(0);
+                    const bool lo = ph + i < 8;
+                    s_w[lane * 8 + ((ph + i) & 7)] = Pair<T>{lo ? v : T(0), lo ? T(0) : v};
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    s_cky[lane * 8 + i] =
+                        i < NS ? Pair<T>{cr * ky[i], ci * ky[i]} : Pair<T>{T(0), T(0)};
+                key = (pts.l0[1][p] << 3) | (X >> 3);
+            }
+            const int prev = __shfl_up(key, 1);
+            const uint64_t starts = __ballot(lane < nb && (lane == 0 || key != prev));
+            __syncthreads();
+            uint64_t mask = starts;
+            while (mask) {
+                const int q0 = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const int q1 = mask ? __builtin_ctzll(mask) : nb;
+                const int gkey = __builtin_amdgcn_readlane(key, q0);
+                const int Y = gkey >> 3;
+                while (R < Y) {
+                    flush(R);
+                    ++R;
+                }
+                const int dy = (r - Y) & 7;
+                switch (gkey & 7) {
+                    case 0: strip8_walk<T, 0>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                    case 1: strip8_walk<T, 1>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                    case 2: strip8_walk<T, 2>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                    case 3: strip8_walk<T, 3>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                    case 4: strip8_walk<T, 4>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                    case 5: strip8_walk<T, 5>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                    case 6: strip8_walk<T, 6>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                    default: strip8_walk<T, 7>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                }
+            }
+            __syncthreads();
+        }
+        // drain the window (and, for a full unit, every remaining row of its tile)
+        const int rstop = heavy ? R + NS : r0 + rows_tile;
+        while (R < rstop) {
+            flush(R);
+            ++R;
+        }
+    }
+}
+
 // fw[y][x] += sum of the scratch tiles of the light units covering (x, y).
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
@@ -250,20 +419,32 @@ template <typename T>
 int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, const T *coef, int kerevalmeth,
                         double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
-                        T *scratch, int64_t scratch_stride, int ntr, hipStream_t stream) {
+                        T *scratch, int64_t scratch_stride, int ntr, int phases,
+                        hipStream_t stream) {
     if (ntr == 0 || pts.M == 0) return JFA_OK;
-    strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces);
-    JFA_HIP_CHECK(hipGetLastError());
+    if (phases & kStripUnits) {
+        strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
     const int64_t ub = sg.nunits + pts.M / sg.pmax + 1;
     const unsigned grid = (unsigned)(ub < 256 * 32 ? ub : 256 * 32);
     const T b = (T)beta;
     const int64_t F = sg.nf0 * sg.nf1;
     JFA_DISPATCH_NS(sg.ns, {
-        spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
-            sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride, fw,
-            fw_stride, scratch, scratch_stride);
-        strip_merge_kernel<T, NS_><<<dim3((unsigned)((F + 255) / 256), ntr), 256, 0, stream>>>(
-            sg, piece_start, scratch, scratch_stride, fw, fw_stride);
+        if (phases & kStripSpread) {
+            if constexpr (NS_ <= 8)
+                spread_strip8_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
+                    sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride,
+                    fw, fw_stride, scratch, scratch_stride);
+            else
+                spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
+                    sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride,
+                    fw, fw_stride, scratch, scratch_stride);
+        }
+        if (phases & kStripMerge)
+            strip_merge_kernel<T, NS_>
+                <<<dim3((unsigned)((F + 255) / 256), ntr), 256, 0, stream>>>(
+                    sg, piece_start, scratch, scratch_stride, fw, fw_stride);
     });
     JFA_HIP_CHECK(hipGetLastError());
     return JFA_OK;
@@ -272,10 +453,10 @@ int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int 
 template int launch_spread_strip<float>(const StripGeom &, const SortedPts<float> &, const int *,
                                         int *, int *, const float *, int, double, const float *,
                                         int64_t, float *, int64_t, float *, int64_t, int,
-                                        hipStream_t);
+                                        int, hipStream_t);
 template int launch_spread_strip<double>(const StripGeom &, const SortedPts<double> &,
                                          const int *, int *, int *, const double *, int, double,
                                          const double *, int64_t, double *, int64_t, double *,
-                                         int64_t, int, hipStream_t);
+                                         int64_t, int, int, hipStream_t);
 
 }  // namespace jfa

@@ -183,6 +183,8 @@ int jfa::setup_strip(jfa_plan p) {
     g.nb[2] = 1;
     g.nbins = (int64_t)g.nb[0] * g.nb[1];
     g.ymajor = 1;
+    // the ns <= 8 spreader also wants each row's points grouped by 8-column block
+    g.keyshift = ns <= 8 ? 3 : 0;
     // buffers
     free_ptr(reinterpret_cast<void *&>(p->d_piece_start));
     JFA_HIP_CHECK(hipMalloc(&p->d_piece_start, sizeof(int) * (sg.nunits + 1)));
@@ -514,10 +516,12 @@ extern "C" int jfa_makeplan(int type, int dim, const int64_t *nmodes, int iflag,
 }
 
 int jfa::default_maxsub(jfa_plan p) {
-    // One subproblem per bin unless a bin is crowded: 4x the mean points per
-    // bin of a uniformly filled grid, at least 1024.
-    (void)p;
-    return 4096;
+    // Spreading: 1024 points per LDS subproblem (cuFINUFFT's published
+    // default).  Each subproblem sums its points into its tile in fp32/fp64
+    // before one global add per cell, so this also bounds the length of the
+    // sequential sums a crowded cell sees.  Interpolation accumulates nothing
+    // across points, and larger subproblems load each tile fewer times.
+    return p->type == 2 ? 4096 : 1024;
 }
 
 // Grow the point-dependent scratch to hold M points.
@@ -630,23 +634,40 @@ BinIndex jfa::bin_view(jfa_plan p) {
 
 // Spread ntr transforms (strengths c with stride c_stride, grids fw with
 // stride fw_stride, fw already zeroed) with the plan's method.
+// With a stage timer, the strip method's three launches are marked as the
+// stages "units", "spread" (the spreading kernel alone) and "merge".
 template <typename T>
-int jfa::spread_any(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t fw_stride, int ntr) {
+static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
+                        int ntr, StageTimer *tm) {
     const SortedPts<T> pts = sorted_view<T>(p);
     const int kem = p->opts.gpu_kerevalmeth ? 1 : 0;
-    if (p->method != JFA_METHOD_STRIP)
-        return launch_spread<T>(p->method, p->geom, pts, bin_view(p), (const T *)p->d_coef, kem,
-                                p->kp.beta, c, c_stride, fw, fw_stride, ntr, p->stream);
+    if (p->method != JFA_METHOD_STRIP) {
+        int rc = launch_spread<T>(p->method, p->geom, pts, bin_view(p), (const T *)p->d_coef, kem,
+                                  p->kp.beta, c, c_stride, fw, fw_stride, ntr, p->stream);
+        if (tm) tm->mark("spread");
+        return rc;
+    }
     for (int t0 = 0; t0 < ntr; t0 += p->batch) {
         const int nb = std::min(p->batch, ntr - t0);
-        int rc = launch_spread_strip<T>(p->sg, pts, p->d_bin_start, p->d_piece_start,
-                                        p->d_npieces, (const T *)p->d_coef, kem, p->kp.beta,
-                                        c + 2 * t0 * c_stride, c_stride, fw + 2 * t0 * fw_stride,
-                                        fw_stride, (T *)p->d_strip_scratch,
-                                        p->strip_scratch_stride, nb, p->stream);
-        if (rc) return rc;
+        const int phases[3] = {kStripUnits, kStripSpread, kStripMerge};
+        const char *names[3] = {"units", "spread", "merge"};
+        for (int ph = 0; ph < 3; ++ph) {
+            if (!tm && ph > 0) break;
+            int rc = launch_spread_strip<T>(
+                p->sg, pts, p->d_bin_start, p->d_piece_start, p->d_npieces, (const T *)p->d_coef,
+                kem, p->kp.beta, c + 2 * t0 * c_stride, c_stride, fw + 2 * t0 * fw_stride,
+                fw_stride, (T *)p->d_strip_scratch, p->strip_scratch_stride, nb,
+                tm ? phases[ph] : kStripAll, p->stream);
+            if (rc) return rc;
+            if (tm) tm->mark(names[ph]);
+        }
     }
     return JFA_OK;
+}
+
+template <typename T>
+int jfa::spread_any(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t fw_stride, int ntr) {
+    return spread_timed<T>(p, c, c_stride, fw, fw_stride, ntr, nullptr);
 }
 template int jfa::spread_any<float>(jfa_plan, const float *, int64_t, float *, int64_t, int);
 template int jfa::spread_any<double>(jfa_plan, const double *, int64_t, double *, int64_t, int);
@@ -670,8 +691,7 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
             if ((rc = launch_fill_zero(fk, sizeof(T) * 2 * p->F * p->ntransf, p->stream)))
                 return rc;
             tm.mark("zero");
-            rc = spread_any<T>(p, c, M, fk, p->F, p->ntransf);
-            tm.mark("spread");
+            rc = spread_timed<T>(p, c, M, fk, p->F, p->ntransf, tm.on ? &tm : nullptr);
         } else {
             rc = launch_interp<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, fk, p->F, c, M,
                                   p->ntransf, p->stream);
@@ -688,8 +708,7 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
         if (p->type == 1) {
             if ((rc = launch_fill_zero(fw, sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
             tm.mark("zero");
-            if ((rc = spread_any<T>(p, cb, M, fw, p->F, nb))) return rc;
-            tm.mark("spread");
+            if ((rc = spread_timed<T>(p, cb, M, fw, p->F, nb, tm.on ? &tm : nullptr))) return rc;
             if ((rc = fft_exec(p, fw, nb))) return rc;
             tm.mark("fft");
             if ((rc = launch_deconv<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2],
