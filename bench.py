@@ -109,6 +109,148 @@ def sampled_accuracy(pts, c, fk, n_sample=48, seed=11):
     return float(torch.linalg.norm(got - exact) / torch.linalg.norm(exact))
 
 
+def grad_ratio(dev, steps=10, warmup=2):
+    """Backward / forward time of loss = sum |nufft2(f, x, y) - d|^2 in 2D:
+    f complex64 512x512 with f_k = exp(-|k|^2 / (2 * 64^2)), M = 2000^2 = 4e6
+    points (a 2000 x 2000 grid of spacing h = 2 pi / 2000, each point jittered
+    by U(-h/2, h/2), wrapped), d complex normal, eps = 1e-6; gradient w.r.t. f
+    and both coordinate arrays.  "fused" is the default reverse mode (analytic
+    kernel-derivative interpolation), "plain" the reference's rule of stacked
+    type-2 transforms weighted by i*k_d (Opts(gpu_fused_adjoint=False))."""
+    import torch
+
+    from jax_finufft_amd import nufft2
+    from jax_finufft_amd.options import Opts
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(6)
+    n = 512
+    k = torch.arange(-n // 2, n // 2, device=dev, dtype=torch.float32)
+    f0 = torch.exp(-(k[:, None] ** 2 + k[None, :] ** 2) / (2 * 64.0**2)).to(torch.complex64)
+    side = 2000
+    h = 2 * math.pi / side
+    grid = -math.pi + h * torch.arange(side, device=dev, dtype=torch.float64)
+    gx, gy = torch.meshgrid(grid, grid, indexing="ij")
+    pts = []
+    for base in (gx.reshape(-1), gy.reshape(-1)):
+        jit = (torch.rand(base.numel(), generator=g, device=dev, dtype=torch.float64) - 0.5) * h
+        v = torch.remainder(base + jit + math.pi, 2 * math.pi) - math.pi
+        pts.append(v.to(torch.float32))
+    M = pts[0].numel()
+    d = torch.complex(torch.randn(M, generator=g, device=dev), torch.randn(M, generator=g, device=dev))
+    out = {"config": "2D type-2, complex64, 512x512 Gaussian modes, M=4e6 jittered-grid points, "
+                     "eps=1e-6; loss = sum|nufft2(f,x,y) - d|^2; grads w.r.t. f, x, y"}
+    for name, fused in (("fused", True), ("plain", False)):
+        opts = Opts(gpu_fused_adjoint=fused)
+        fwd_ms, bwd_ms = [], []
+        for it in range(warmup + steps):
+            f = f0.clone().requires_grad_(True)
+            x = pts[0].clone().requires_grad_(True)
+            y = pts[1].clone().requires_grad_(True)
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            torch.cuda.synchronize()
+            e[0].record()
+            loss = (nufft2(f, x, y, eps=1e-6, opts=opts) - d).abs().pow(2).sum()
+            e[1].record()
+            loss.backward()
+            e[2].record()
+            torch.cuda.synchronize()
+            if it >= warmup:
+                fwd_ms.append(e[0].elapsed_time(e[1]))
+                bwd_ms.append(e[1].elapsed_time(e[2]))
+        fm = sorted(fwd_ms)[len(fwd_ms) // 2]
+        bm = sorted(bwd_ms)[len(bwd_ms) // 2]
+        out[name] = {"fwd_ms": round(fm, 4), "bwd_ms": round(bm, 4), "bwd_over_fwd": round(bm / fm, 3)}
+    return out
+
+
+def ragged_batch(dev, steps=5, warmup=2):
+    """64 independent 2D type-1 problems: M_i = round(10^U(4, log10 2e6)),
+    modes N_i x N_i with N_i uniform in {64, 128, 256, 512}, T_i uniform in
+    {1..8} stacked complex64 strength vectors sharing the points, eps = 1e-6,
+    numpy default_rng(7).  Times one ragged plan run (sort, spread, FFT and
+    deconvolution of every problem; graph-replayed) against a Python loop of
+    nufft1 calls (plans come from the per-shape cache, as for any caller) and
+    against a loop over 64 pre-made guru plans (setpts + execute each)."""
+    import numpy as np
+    import torch
+
+    from jax_finufft_amd import Plan, RaggedPlan, nufft1
+
+    rng = np.random.default_rng(7)
+    nprob = 64
+    Ms = np.round(10 ** rng.uniform(4, math.log10(2e6), nprob)).astype(int)
+    Ns = rng.choice([64, 128, 256, 512], nprob)
+    Ts = rng.integers(1, 9, nprob)
+    shapes = [(int(n), int(n)) for n in Ns]
+    xs, ys, cs = [], [], []
+    for m, t in zip(Ms, Ts):
+        xs.append(((torch.rand(int(m), device=dev) * 2 - 1) * math.pi).contiguous())
+        ys.append(((torch.rand(int(m), device=dev) * 2 - 1) * math.pi).contiguous())
+        cs.append(torch.complex(torch.randn(int(t), int(m), device=dev),
+                                torch.randn(int(t), int(m), device=dev)))
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        return max(e0.elapsed_time(e1) / steps, wall)
+
+    plan = RaggedPlan(shapes, [int(m) for m in Ms], [int(t) for t in Ts], eps=1e-6, graph=True)
+    outs = [torch.empty((int(t), n, n), dtype=torch.complex64, device=dev)
+            for t, (n, _) in zip(Ts, shapes)]
+    ragged_ms = timed(lambda: plan.run(xs, ys, cs, out=outs))
+
+    def loop_calls():
+        for i in range(nprob):
+            nufft1(shapes[i], cs[i], xs[i], ys[i], eps=1e-6)
+
+    loop_ms = timed(loop_calls)
+
+    guru = [Plan(1, shapes[i], iflag=1, ntransf=int(Ts[i]), eps=1e-6, device=dev)
+            for i in range(nprob)]
+
+    def loop_guru():
+        for i in range(nprob):
+            guru[i].setpts(xs[i], ys[i])
+            guru[i].execute(cs[i], outs[i])
+
+    guru_ms = timed(loop_guru)
+    # agreement of the ragged outputs with the single calls (fp32 rounding)
+    plan.run(xs, ys, cs, out=outs)
+    worst = 0.0
+    for i in range(0, nprob, 8):
+        ref = nufft1(shapes[i], cs[i], xs[i], ys[i], eps=1e-6)
+        worst = max(worst, float(torch.linalg.norm(outs[i] - ref) / torch.linalg.norm(ref)))
+    for g in guru:
+        g.destroy()
+    uses_graph = plan.uses_graph
+    plan.destroy()
+    total_pts = int(Ms.sum())
+    return {
+        "config": "64 2D type-1 problems, M log-uniform in [1e4, 2e6], N in {64..512}^2, "
+                  "ntransf 1-8, complex64, eps=1e-6, seed 7",
+        "total_points": total_pts,
+        "total_point_transforms": int((Ms * Ts).sum()),
+        "ragged_ms": round(ragged_ms, 3),
+        "loop_of_nufft1_ms": round(loop_ms, 3),
+        "loop_of_guru_plans_ms": round(guru_ms, 3),
+        "speedup_vs_nufft1_loop": round(loop_ms / ragged_ms, 2),
+        "speedup_vs_guru_loop": round(guru_ms / ragged_ms, 2),
+        "graph_replayed": uses_graph,
+        "max_rel_l2_vs_single_calls": worst,
+    }
+
+
 def traffic_from_profiles():
     """HBM bytes per spread launch from committed rocprofv3 --pmc summaries."""
     path = os.path.join(ROOT, "profiles", "spread_2d_t1_f32_traffic.json")
@@ -157,6 +299,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--no-clustered", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this backend has no multi-GPU path, so --gpus must "
@@ -237,6 +381,11 @@ def main():
             "points": "Gaussian sigma=0.1 about the origin, wrapped",
         }
         del cplan
+
+    if not args.no_grad:
+        out["grad"] = grad_ratio(dev)
+    if not args.no_ragged:
+        out["ragged"] = ragged_batch(dev)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, info["ns"])

@@ -134,6 +134,38 @@ int jfa_plan_timing(jfa_plan plan, const char *stage, int reset, double *total_m
 
 const char *jfa_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * Ragged batches (no reference counterpart; the reference's equivalent is a
+ * Python loop of nufft1 calls, each a run_nufft with its own plan,
+ * lib/kernels.cc.cu:25-92).  nprob independent 2D type-1 problems with their
+ * own point counts, mode grids and numbers of stacked strength vectors, all
+ * with one eps and iflag, run as one sequence of launches: one point sort over
+ * every problem, one strip-spreading launch over every problem's units, one
+ * merge, one batched FFT per fine-grid size, one deconvolution.  With
+ * use_graph != 0 the launches after the sort are captured once into a HIP
+ * graph and replayed on later runs.
+ *   nmodes:   nprob x dim mode counts, x fastest (the backend's axis order)
+ *   ntransf:  nprob stacked strength vectors per problem
+ *   npoints:  nprob point counts
+ * opts: modeord, upsampfac, width_rule, kernel_width, gpu_kerevalmeth,
+ * gpu_maxsubprobsize, gpu_stream and gpu_device_id are honoured.  dim must be
+ * 2 and every fine grid at least 64 x 2*ns (JFA_ERR_NOT_SUPPORTED otherwise).
+ * ------------------------------------------------------------------------ */
+typedef struct jfa_ragged_s *jfa_ragged;
+int jfa_ragged_makeplan(int dim, int nprob, const int64_t *nmodes, const int64_t *ntransf,
+                        const int64_t *npoints, int iflag, double eps, int is_double,
+                        const jfa_opts *opts, int use_graph, jfa_ragged *plan_out);
+/* points: dim x nprob device pointers (x of every problem, then y); c, f:
+ * nprob device pointers each, c[i] holding ntransf[i] x npoints[i] complex
+ * strengths and f[i] receiving ntransf[i] x modes complex outputs.  Queues
+ * setpts + execute of every problem on the plan's stream; no host wait. */
+int jfa_ragged_run(jfa_ragged plan, const void *const *points, const void *const *c,
+                   void *const *f);
+/* 1 if runs replay a captured graph, 0 if they launch eagerly. */
+int jfa_ragged_uses_graph(jfa_ragged plan);
+int jfa_ragged_set_stream(jfa_ragged plan, jfa_stream_t stream);
+int jfa_ragged_destroy(jfa_ragged plan);
+
 /* Number of plans held by the per-device plan cache, and a way to drop them. */
 int jfa_cache_size(void);
 void jfa_cache_clear(void);

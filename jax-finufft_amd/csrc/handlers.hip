@@ -50,7 +50,7 @@ struct Entry {
 std::mutex g_mu;
 std::vector<Entry> g_cache;
 uint64_t g_clock = 0;
-constexpr size_t kMaxIdlePlans = 16;
+constexpr size_t kMaxIdlePlans = 32;
 
 jfa_plan checkout(const Key &k, const jfa_opts &o, int *rc) {
     {

@@ -75,7 +75,7 @@ struct GridGeom {
     int ns;              // kernel width
     int dim;
     int ymajor;          // 2D bins ordered y fastest (bin = by + nb1*bx), for strips
-    int keyshift;        // sort key = bin << keyshift | ((l0x mod bs0) >> 3); 0 = bin only
+    int keyshift;        // sort key = bin << keyshift | (l0x mod bs0); 0 = bin only
 };
 
 // Geometry of the 2D strip spreader (kernels_strip.hip): strips of SW = 64 -
@@ -89,6 +89,47 @@ struct StripGeom {
     int pmax;       // points per piece of a crowded unit
 };
 
+// Strip geometry for a 2D grid g (nf and ns set): units of H rows sized for
+// about 4096 units, H in [ns, nf1 - ns + 1] so that a cell is covered by at
+// most two units per axis; also sets g's bins to (strip, row), rows fastest.
+StripGeom strip_geometry(GridGeom &g, int pmax);
+int ensure_rocfft();
+
+// Ragged batch of 2D type-1 problems (ragged.hip).  Per problem: its grid and
+// bins for setpts.  Per virtual problem (problem, transform): what spreading,
+// merging and deconvolution need.  Offsets count complex cells.
+struct RaggedProb {
+    GridGeom g;
+    int64_t moff;     // first point in the concatenated point order
+    int64_t bin_off;  // first global bin
+};
+struct RaggedVirt {
+    StripGeom sg;
+    int prob, t;
+    int64_t m;         // the problem's point count
+    int64_t bin_off;   // the problem's first global bin
+    int64_t fw_off;    // into the concatenated fine grids
+    int64_t tile_off;  // into the concatenated scratch tiles
+    int unit_off;      // first global unit
+    int64_t n0, n1;    // modes, x fastest
+    int64_t dk_off;    // 1/psi_hat: dk0 at dk + dk_off, dk1 at dk + dk_off + n0
+    int64_t moff;      // the problem's first point in the concatenated order
+                       // (sorted indices are concatenated-order indices)
+};
+// Device tables and sizes of one ragged launch sequence.
+struct RaggedLaunch {
+    const RaggedProb *probs;
+    const RaggedVirt *virt;
+    const int64_t *moff;       // nprob + 1
+    const int *vunit_off;      // nv + 1
+    const int64_t *cell_blk;   // nv + 1: first merge block (256 cells) per virtual problem
+    const int64_t *mode_blk;   // nv + 1: first deconv block (256 modes)
+    const void *const *ptrs;   // x[nprob], y[nprob], c[nprob], f[nprob]
+    int *bin_start, *piece_start, *npieces;
+    int nprob, nv, ns, pmax, keyshift;
+    int64_t mtot, nbins, nunits, cell_blocks, mode_blocks;
+};
+
 // Points after setpts, in bin-sorted order: original index, first tap
 // (wrapped into [0, nf)) and the Horner variable t = 2*frac - 1 per dim.
 template <typename T>
@@ -97,6 +138,21 @@ struct SortedPts {
     const int *idx;
     const int *l0[3];
     const T *t[3];
+};
+
+// Points as the 2D strip spreader reads them after its setpts: the sorted keys
+// and original indices, and every point's Horner variables (tx, ty) in input
+// order, so that setpts needs no gather.  key = (bin << kStripKeyShift) | X with
+// bin = bin_off + s*nf1 + l0y (s the strip) and X = l0x - s*SW in [0, SW).
+constexpr int kStripKeyShift = 6;
+template <typename T>
+struct StripPts {
+    int64_t M;
+    const int *idx;       // sorted position -> input point P (over all problems)
+    const unsigned *key;  // sorted keys
+    const T *rec;         // (tx, ty) of input point P at rec[2*P]
+    int64_t bin_off;      // the problem's first global bin (0 for one plan)
+    int64_t idx_off;      // the problem's first input point: c index = P - idx_off
 };
 
 // Per-bin bookkeeping for the LDS-subproblem method.
@@ -127,19 +183,38 @@ template <typename T>
 int launch_fold_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
                        int *idx, int *l0, T *tv, hipStream_t stream);
 int launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
+// Setpts of the 2D strip spreader: keys (with the first-tap column, see
+// StripPts) radix-sorted with the original indices, bin_start, and the Horner
+// variables written in input order to rec; no gather.
+template <typename T>
+int launch_sort_points_strip(const GridGeom &g, int64_t M, const T *x, const T *y,
+                             const SortScratch &s, int *bin_start, int *idx, T *rec,
+                             hipStream_t stream);
 
 template <typename T>
 int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
                   const T *coef, int kerevalmeth, double beta, const T *c, int64_t c_stride,
                   T *fw, int64_t fw_stride, int ntr, hipStream_t stream);
 template <typename T>
-int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int *bin_start,
+int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, const T *coef, int kerevalmeth,
                         double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
                         T *scratch, int64_t scratch_stride, int ntr, int phases,
                         hipStream_t stream);
 // phases of launch_spread_strip (a bit mask): piece table, spreading, merge
 constexpr int kStripUnits = 1, kStripSpread = 2, kStripMerge = 4, kStripAll = 7;
+// Ragged batch: sort every problem's points at once (global bins), then the
+// strip phases over every virtual problem's units, then deconvolution.
+template <typename T>
+int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T *rec,
+                       hipStream_t stream);
+template <typename T>
+int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T *coef,
+                        int kerevalmeth, double beta, T *fw_all, T *scratch_all, int phases,
+                        hipStream_t stream);
+template <typename T>
+int launch_ragged_deconv(const RaggedLaunch &rl, const T *dk, int modeord, const T *fw_all,
+                         hipStream_t stream);
 template <typename T>
 int launch_interp(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
                   const T *coef, int kerevalmeth, double beta, const T *fw, int64_t fw_stride,

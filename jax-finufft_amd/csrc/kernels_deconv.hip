@@ -101,7 +101,51 @@ __global__ void __launch_bounds__(256)
     fw[2 * l + 1] = vi;
 }
 
+// Ragged batch: block b deconvolves 256 modes of the virtual problem v with
+// mode_blk[v] <= b; the output of problem i is f_i (pointer table entry
+// 3*nprob + i), transform-major.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    ragged_deconv_kernel(const RaggedVirt *__restrict__ virt, const int64_t *__restrict__ mode_blk,
+                         int nv, const T *__restrict__ dk, int modeord,
+                         const T *__restrict__ fw_all, const void *const *__restrict__ ptrs,
+                         int nprob) {
+    int lo = 0, hi = nv;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (mode_blk[mid] <= (int64_t)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const RaggedVirt d = virt[lo];
+    const int64_t q = ((int64_t)blockIdx.x - mode_blk[lo]) * 256 + threadIdx.x;
+    const int64_t n0 = d.n0, n1 = d.n1;
+    if (q >= n0 * n1) return;
+    const int64_t nf0 = d.sg.nf0, nf1 = d.sg.nf1;
+    const T *dk0 = dk + d.dk_off;
+    const T *dk1 = dk0 + n0;
+    const int64_t k0 = freq_of(q % n0, n0, modeord);
+    const int64_t k1 = freq_of(q / n0, n1, modeord);
+    const T s = dk0[k0 + n0 / 2] * dk1[k1 + n1 / 2];
+    const int64_t l = (k0 < 0 ? k0 + nf0 : k0) + nf0 * (k1 < 0 ? k1 + nf1 : k1);
+    const T *fw = fw_all + 2 * d.fw_off;
+    T *out = (T *)ptrs[3 * nprob + d.prob] + 2 * (int64_t)d.t * n0 * n1;
+    out[2 * q] = fw[2 * l] * s;
+    out[2 * q + 1] = fw[2 * l + 1] * s;
+}
+
 }  // namespace
+
+template <typename T>
+int launch_ragged_deconv(const RaggedLaunch &rl, const T *dk, int modeord, const T *fw_all,
+                         hipStream_t stream) {
+    if (rl.mode_blocks == 0) return JFA_OK;
+    ragged_deconv_kernel<T><<<(unsigned)rl.mode_blocks, 256, 0, stream>>>(
+        rl.virt, rl.mode_blk, rl.nv, dk, modeord, fw_all, rl.ptrs, rl.nprob);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
 
 template <typename T>
 int launch_deconv(int dim, const int64_t *n, const int64_t *nf, const T *dk0, const T *dk1,
@@ -132,6 +176,8 @@ int launch_amplify(int dim, const int64_t *n, const int64_t *nf, const T *dk0, c
 }
 
 #define JFA_INST(T)                                                                           \
+    template int launch_ragged_deconv<T>(const RaggedLaunch &, const T *, int, const T *,      \
+                                         hipStream_t);                                         \
     template int launch_deconv<T>(int, const int64_t *, const int64_t *, const T *, const T *, \
                                   const T *, int, const T *, int64_t, T *, int64_t, int,       \
                                   hipStream_t);                                                \

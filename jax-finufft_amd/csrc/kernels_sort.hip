@@ -11,6 +11,11 @@
 //              first tap and Horner variable per dimension at p (coalesced
 //              writes; the coordinates are read by index)
 //
+// The 2D strip spreader skips the gather: its key also carries the first tap's
+// column within the strip (low kStripKeyShift bits), which with the bin gives
+// both first taps, and the keys kernel writes the Horner variables in input
+// order, where the spreader reads them by original index.
+//
 // The bin of a point and the tap window the spreader uses both come from
 // fold_point() on the coordinate in double, so they agree exactly.
 #include <rocprim/device/device_radix_sort.hpp>
@@ -21,14 +26,12 @@ namespace jfa {
 
 namespace {
 
-// Sort key of point j: its bin, and with keyshift = 3 (the 2D strip spreader)
-// also the 8-column block of its first tap within the strip as the low bits.
+// Sort key of point j: its bin.
 template <typename T, int DIM>
 __device__ __forceinline__ unsigned key_of_point(const GridGeom &g, const T *x, const T *y,
                                                  const T *z, int64_t j) {
     Folded f0 = fold_point((double)x[j], g.nf[0], g.scale[0], g.shift[0], g.ns);
     int64_t b = f0.l0w / g.bs[0];
-    const int sub = (f0.l0w - (int)b * g.bs[0]) >> 3;
     if constexpr (DIM > 1) {
         Folded f1 = fold_point((double)y[j], g.nf[1], g.scale[1], g.shift[1], g.ns);
         if (DIM == 2 && g.ymajor)
@@ -40,7 +43,35 @@ __device__ __forceinline__ unsigned key_of_point(const GridGeom &g, const T *x, 
         Folded f2 = fold_point((double)z[j], g.nf[2], g.scale[2], g.shift[2], g.ns);
         b += (int64_t)(f2.l0w / g.bs[2]) * g.nb[0] * g.nb[1];
     }
-    return g.keyshift ? (unsigned)((b << g.keyshift) | sub) : (unsigned)b;
+    return (unsigned)b;
+}
+
+// Strip key of 2D point j: ((bin_off + s*nf1 + l0y) << kStripKeyShift) | X
+// with s = l0x / SW and X = l0x - s*SW, and its Horner variables.
+template <typename T>
+__device__ __forceinline__ unsigned strip_key(const GridGeom &g, int64_t bin_off, const T *x,
+                                              const T *y, int64_t j, T &tx, T &ty) {
+    const Folded f0 = fold_point((double)x[j], g.nf[0], g.scale[0], g.shift[0], g.ns);
+    const Folded f1 = fold_point((double)y[j], g.nf[1], g.scale[1], g.shift[1], g.ns);
+    const int s = f0.l0w / g.bs[0];
+    const int X = f0.l0w - s * g.bs[0];
+    tx = (T)f0.t;
+    ty = (T)f1.t;
+    const int64_t b = bin_off + (int64_t)s * g.nb[1] + f1.l0w;
+    return (unsigned)((b << kStripKeyShift) | X);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    strip_keys_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
+                      unsigned *key, unsigned *val, T *rec) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    T tx, ty;
+    key[j] = strip_key<T>(g, 0, x, y, j, tx, ty);
+    val[j] = (unsigned)j;
+    rec[2 * j] = tx;
+    rec[2 * j + 1] = ty;
 }
 
 template <typename T, int DIM>
@@ -141,6 +172,40 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// ---------------------------------------------------------------- ragged batch
+// Global point P belongs to problem i with moff[i] <= P < moff[i+1]; its
+// coordinates are x_i[P - moff[i]], y_i[...] from the pointer table.  Keys are
+// offset by the problem's first global bin, so one sort orders every problem.
+
+__device__ __forceinline__ int problem_of(const int64_t *__restrict__ moff, int nprob, int64_t P) {
+    int lo = 0, hi = nprob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (moff[mid] <= P)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    ragged_keys_kernel(const RaggedProb *__restrict__ probs, const int64_t *__restrict__ moff,
+                       int nprob, const void *const *__restrict__ ptrs, int64_t mtot,
+                       unsigned *key, unsigned *val, T *rec) {
+    const int64_t P = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (P >= mtot) return;
+    const int i = problem_of(moff, nprob, P);
+    const RaggedProb &pr = probs[i];
+    T tx, ty;
+    key[P] = strip_key<T>(pr.g, pr.bin_off, (const T *)ptrs[i], (const T *)ptrs[nprob + i],
+                          P - pr.moff, tx, ty);
+    val[P] = (unsigned)P;
+    rec[2 * P] = tx;
+    rec[2 * P + 1] = ty;
+}
+
 __global__ void fill_zero_kernel(uint4 *p, int64_t n16, unsigned char *tail, int ntail) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -229,7 +294,53 @@ int launch_fold_points(const GridGeom &g, int64_t M, const T *x, const T *y, con
     return JFA_OK;
 }
 
+// Shared tail of the strip sorts: radix sort of (key, index) and bin bounds.
+int sort_strip_keys(int64_t M, int64_t nbins, const SortScratch &s, int *bin_start, int *idx,
+                    hipStream_t stream) {
+    if (M > 0) {
+        size_t bytes = s.temp_bytes;
+        JFA_HIP_CHECK(rocprim::radix_sort_pairs(s.temp, bytes, s.key_in, s.key_out, s.val_in,
+                                                (unsigned *)idx, (unsigned)M, 0,
+                                                key_bits(nbins) + kStripKeyShift, stream));
+    }
+    const unsigned blocks = (unsigned)((nbins + 1 + 255) / 256);
+    bin_bounds_kernel<<<blocks, 256, 0, stream>>>(M, nbins, kStripKeyShift, s.key_out,
+                                                  bin_start);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
+int launch_sort_points_strip(const GridGeom &g, int64_t M, const T *x, const T *y,
+                             const SortScratch &s, int *bin_start, int *idx, T *rec,
+                             hipStream_t stream) {
+    if (M > 0) {
+        const unsigned blocks = (unsigned)((M + 255) / 256);
+        strip_keys_kernel<T><<<blocks, 256, 0, stream>>>(g, M, x, y, s.key_in, s.val_in, rec);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    return sort_strip_keys(M, g.nbins, s, bin_start, idx, stream);
+}
+
+template <typename T>
+int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T *rec,
+                       hipStream_t stream) {
+    const int64_t M = rl.mtot;
+    if (M > 0) {
+        const unsigned blocks = (unsigned)((M + 255) / 256);
+        ragged_keys_kernel<T><<<blocks, 256, 0, stream>>>(rl.probs, rl.moff, rl.nprob, rl.ptrs, M,
+                                                          s.key_in, s.val_in, rec);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    return sort_strip_keys(M, rl.nbins, s, rl.bin_start, idx, stream);
+}
+
 #define JFA_INST(T)                                                                          \
+    template int launch_ragged_sort<T>(const RaggedLaunch &, const SortScratch &, int *, T *, \
+                                       hipStream_t);                                         \
+    template int launch_sort_points_strip<T>(const GridGeom &, int64_t, const T *, const T *, \
+                                             const SortScratch &, int *, int *, T *,          \
+                                             hipStream_t);                                    \
     template int launch_sort_points<T>(const GridGeom &, int64_t, const T *, const T *,      \
                                        const T *, const SortScratch &, int *, int *, int *,  \
                                        int, int *, int *, T *, hipStream_t);                 \

@@ -12,17 +12,16 @@
 // [s*SW, s*SW + 64).  Points are sorted by (strip, l0y) (the "y-major" bins
 // set up by the plan; for ns <= 8 also by 8-column block within the row), so
 // within a strip they arrive in row order.  A strip is cut into units of H
-// consecutive l0y rows.
+// consecutive l0y rows, and one wave spreads one unit.
 //
-// For ns <= 8, spread_strip8_kernel below assigns cells to lanes modulo 8 in
-// both directions, so that every lane does useful work for every point.  For
-// wider kernels, one wave per unit: lane x owns column s*SW + x and holds a
-// rolling window of
-// ns rows (real and imaginary parts) in registers.  Points are staged 64 at a
-// time: each lane evaluates one point's ns x-weights and ns y-weights (Horner)
-// and writes the x-weights and c*y-weights to wave-private LDS; then the wave
-// walks the batch in order, every lane reading its own x-weight (zero outside
-// the point's 8-wide window) and the point's y-weights by broadcast, and doing
+// ns <= 8 (strip8_piece): cells are assigned to lanes modulo 8 in both
+// directions, so that every lane does useful work for every point (below).
+// Wider kernels (wide_piece): lane x owns column s*SW + x and holds a rolling
+// window of ns rows (real and imaginary parts) in registers.  Points are staged
+// 64 at a time: each lane evaluates one point's ns x-weights and ns y-weights
+// (Horner) and writes the x-weights and c*y-weights to wave-private LDS; then
+// the wave walks the batch in order, every lane reading its own x-weight (zero
+// outside the point's window) and the point's y-weights by broadcast, and doing
 // 2*ns FMAs into its window.  When the next point starts a later row, the
 // window's first row is complete and is written out.  No atomics touch LDS.
 //
@@ -32,6 +31,10 @@
 // than `pmax` points is split into pieces processed by separate waves; those
 // pieces add their rows to the (zeroed) fine grid with global atomics instead,
 // and the merge skips their tiles.
+//
+// The same per-piece and per-cell bodies serve one plan (a single problem,
+// blockIdx.y = transform) and a ragged batch of problems (descriptor tables,
+// one launch over every problem's units).
 #include "jfa_device.hpp"
 
 namespace jfa {
@@ -39,13 +42,13 @@ namespace {
 
 constexpr int kStripLanes = 64;
 
-// unit -> (strip, chunk) and its point range; piece -> unit by binary search
-__device__ __forceinline__ int piece_unit(const int *__restrict__ piece_start, int nunits,
-                                          int piece) {
-    int lo = 0, hi = nunits;
+// first index i in [0, n) with starts[i] <= v < starts[i + 1]
+template <typename I>
+__device__ __forceinline__ int segment_of(const I *__restrict__ starts, int n, int64_t v) {
+    int lo = 0, hi = n;
     while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (piece_start[mid] <= piece)
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)starts[mid] <= v)
             lo = mid;
         else
             hi = mid;
@@ -53,136 +56,160 @@ __device__ __forceinline__ int piece_unit(const int *__restrict__ piece_start, i
     return lo;
 }
 
-template <typename T, int NS>
-__global__ void __launch_bounds__(kStripLanes)
-    spread_strip_kernel(StripGeom sg, SortedPts<T> pts, const int *__restrict__ bin_start,
-                        const int *__restrict__ piece_start, const int *__restrict__ npieces,
-                        const T *__restrict__ coef, T beta, int horner, const T *__restrict__ c,
-                        int64_t c_stride, T *__restrict__ fw, int64_t fw_stride,
-                        T *__restrict__ scratch, int64_t scratch_stride) {
-    __shared__ T s_kx[kStripLanes * NS];
-    __shared__ T s_cky[kStripLanes * NS * 2];
-    __shared__ int s_lx[kStripLanes];
-    __shared__ int s_ly[kStripLanes];
-    c += 2 * c_stride * blockIdx.y;
-    fw += 2 * fw_stride * blockIdx.y;
-    scratch += 2 * scratch_stride * blockIdx.y;
-    const int lane = threadIdx.x;
-    const int total = *npieces;
-    const int rows_tile = sg.H + NS - 1;
-    for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
-        const int u = piece_unit(piece_start, sg.nunits, piece);
-        const int sub = piece - piece_start[u];
-        const bool heavy = piece_start[u + 1] - piece_start[u] > 1;
-        const int s = u / sg.K, k = u - s * sg.K;
-        const int ox = s * sg.SW;
-        const int r0 = k * sg.H;
-        const int r1 = min(r0 + sg.H, (int)sg.nf1);
-        const int ub = bin_start[(int64_t)s * sg.nf1 + r0];
-        const int ue = bin_start[(int64_t)s * sg.nf1 + r1];
-        const int pb = heavy ? ub + sub * sg.pmax : ub;
-        const int pe = heavy ? min(pb + sg.pmax, ue) : ue;
-        T acc_r[NS], acc_i[NS];
-#pragma unroll
-        for (int b = 0; b < NS; ++b) acc_r[b] = acc_i[b] = T(0);
-        // rolling window: acc[b] holds row rcur + b
-        int rcur = heavy ? (pe > pb ? pts.l0[1][pb] : r0) : r0;
-        T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
-        auto flush = [&](int row) {
-            if (heavy) {
-                int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
-                int64_t x = ox + lane;
-                if (x >= sg.nf0) x -= sg.nf0;
-                const int64_t gi = y * sg.nf0 + x;
-                if (acc_r[0] != T(0) || acc_i[0] != T(0)) {
-                    atomic_add_global(&fw[2 * gi], acc_r[0]);
-                    atomic_add_global(&fw[2 * gi + 1], acc_i[0]);
-                }
-            } else {
-                const int64_t q = (int64_t)(row - r0) * kStripLanes + lane;
-                tile[2 * q] = acc_r[0];
-                tile[2 * q + 1] = acc_i[0];
-            }
-#pragma unroll
-            for (int b = 0; b + 1 < NS; ++b) {
-                acc_r[b] = acc_r[b + 1];
-                acc_i[b] = acc_i[b + 1];
-            }
-            acc_r[NS - 1] = acc_i[NS - 1] = T(0);
-        };
-        for (int bb = pb; bb < pe; bb += kStripLanes) {
-            const int nb = min(kStripLanes, pe - bb);
-            // stage: lane q evaluates point bb + q
-            if (lane < nb) {
-                const int p = bb + lane;
-                const int64_t j = pts.idx[p];
-                const T cr = c[2 * j], ci = c[2 * j + 1];
-                T kx[NS], ky[NS];
-                if (horner) {
-                    eval_kernel<T, NS, true>(coef, beta, pts.t[0][p], kx);
-                    eval_kernel<T, NS, true>(coef, beta, pts.t[1][p], ky);
-                } else {
-                    eval_kernel<T, NS, false>(coef, beta, pts.t[0][p], kx);
-                    eval_kernel<T, NS, false>(coef, beta, pts.t[1][p], ky);
-                }
-#pragma unroll
-                for (int a = 0; a < NS; ++a) s_kx[lane * NS + a] = kx[a];
-#pragma unroll
-                for (int b = 0; b < NS; ++b) {
-                    s_cky[(lane * NS + b) * 2] = cr * ky[b];
-                    s_cky[(lane * NS + b) * 2 + 1] = ci * ky[b];
-                }
-                s_lx[lane] = pts.l0[0][p] - ox;
-                s_ly[lane] = pts.l0[1][p];
-            }
-            __syncthreads();
-            for (int q = 0; q < nb; ++q) {
-                const int ly = s_ly[q];
-                while (rcur < ly) {
-                    flush(rcur);
-                    ++rcur;
-                }
-                const int kk = lane - s_lx[q];
-                const bool in = kk >= 0 && kk < NS;
-                const T kxv = in ? s_kx[q * NS + (in ? kk : 0)] : T(0);
-                const T *cy = &s_cky[q * NS * 2];
-#pragma unroll
-                for (int b = 0; b < NS; ++b) {
-                    acc_r[b] = fma(cy[2 * b], kxv, acc_r[b]);
-                    acc_i[b] = fma(cy[2 * b + 1], kxv, acc_i[b]);
-                }
-            }
-            __syncthreads();
-        }
-        // drain the window (and, for a full unit, every remaining row of its tile)
-        const int rstop = heavy ? rcur + NS : r0 + rows_tile;
-        while (rcur < rstop) {
-            flush(rcur);
-            ++rcur;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Modular-ownership variant for ns <= 8 (the fp32 widths).  The unit, its
-// pieces, its scratch tile and the merge are those of spread_strip_kernel; what
-// changes is which lane holds which cell.  Lane (a, r) = (lane & 7, lane >> 3)
-// holds the 8 cells of one row whose columns are a, a+8, ..., a+56 of the strip
-// (accumulator k is column 8k + a), for the window row congruent to r mod 8.
-// An ns x ns footprint then puts exactly one cell in every lane (ns = 8) and
-// each lane adds one complex product per point instead of ns.
-//
-// Which of its eight accumulators a lane adds to depends on the point's 8-column
-// block bx = X >> 3 (X = first tap within the strip) and on whether the lane's
-// column a lies before the point's phase X & 7 (then it is the next block).
-// The first is made wave-uniform by sorting the points by (row, bx) (sort key
-// low bits) and walking them in runs of equal (row, bx) under a switch; the
-// second by staging, per point and lane column, the weight split into a pair
-// (weight in block bx, weight in block bx + 1), one of which is zero.
 template <typename T>
 struct alignas(2 * sizeof(T)) Pair {
     T x, y;
 };
+
+// LDS staging of one wave: ns <= 8 uses the weight pairs, wider kernels the
+// column weights and c * y-weights per point.
+template <typename T, int NS, bool MOD8 = (NS <= 8)>
+struct StripLds;
+template <typename T, int NS>
+struct StripLds<T, NS, true> {
+    Pair<T> w[kStripLanes * 8];    // per point q, lane column a: (weight in bx, in bx+1)
+    Pair<T> cky[kStripLanes * 8];  // per point q, row offset dy: c * ky[dy] (0 if dy >= NS)
+};
+template <typename T, int NS>
+struct StripLds<T, NS, false> {
+    T kx[kStripLanes * NS];
+    T cky[kStripLanes * NS * 2];
+    int lx[kStripLanes];
+    int ly[kStripLanes];
+};
+
+// The point range [pb, pe) of piece `sub` of unit u, and its strip / rows.
+struct PieceRange {
+    int ox, r0, pb, pe;
+};
+__device__ __forceinline__ PieceRange piece_range(const StripGeom &sg,
+                                                  const int *__restrict__ bin_start, int u,
+                                                  int sub, bool heavy) {
+    const int s = u / sg.K, k = u - s * sg.K;
+    PieceRange pr;
+    pr.ox = s * sg.SW;
+    pr.r0 = k * sg.H;
+    const int r1 = min(pr.r0 + sg.H, (int)sg.nf1);
+    const int ub = bin_start[(int64_t)s * sg.nf1 + pr.r0];
+    const int ue = bin_start[(int64_t)s * sg.nf1 + r1];
+    pr.pb = heavy ? ub + sub * sg.pmax : ub;
+    pr.pe = heavy ? min(pr.pb + sg.pmax, ue) : ue;
+    return pr;
+}
+
+// ------------------------------------------------------------ ns > 8 body
+
+template <typename T, int NS>
+__device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T> &pts,
+                                           const int *__restrict__ bin_start, int u, int sub,
+                                           bool heavy, const T *__restrict__ coef, T beta,
+                                           int horner, const T *__restrict__ c,
+                                           T *__restrict__ fw, T *__restrict__ scratch,
+                                           StripLds<T, NS> &lds) {
+    const int lane = threadIdx.x;
+    const int rows_tile = sg.H + NS - 1;
+    const PieceRange pr = piece_range(sg, bin_start, u, sub, heavy);
+    const int ox = pr.ox, r0 = pr.r0, pb = pr.pb, pe = pr.pe;
+    // key >> kStripKeyShift = rowbase + l0y for every point of this strip
+    const int64_t rowbase = pts.bin_off + (int64_t)(u / sg.K) * sg.nf1;
+    T acc_r[NS], acc_i[NS];
+#pragma unroll
+    for (int b = 0; b < NS; ++b) acc_r[b] = acc_i[b] = T(0);
+    // rolling window: acc[b] holds row rcur + b
+    int rcur = heavy ? (pe > pb ? (int)((pts.key[pb] >> kStripKeyShift) - rowbase) : r0) : r0;
+    T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
+    auto flush = [&](int row) {
+        if (heavy) {
+            int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
+            int64_t x = ox + lane;
+            if (x >= sg.nf0) x -= sg.nf0;
+            const int64_t gi = y * sg.nf0 + x;
+            if (acc_r[0] != T(0) || acc_i[0] != T(0)) {
+                atomic_add_global(&fw[2 * gi], acc_r[0]);
+                atomic_add_global(&fw[2 * gi + 1], acc_i[0]);
+            }
+        } else {
+            const int64_t q = (int64_t)(row - r0) * kStripLanes + lane;
+            tile[2 * q] = acc_r[0];
+            tile[2 * q + 1] = acc_i[0];
+        }
+#pragma unroll
+        for (int b = 0; b + 1 < NS; ++b) {
+            acc_r[b] = acc_r[b + 1];
+            acc_i[b] = acc_i[b + 1];
+        }
+        acc_r[NS - 1] = acc_i[NS - 1] = T(0);
+    };
+    for (int bb = pb; bb < pe; bb += kStripLanes) {
+        const int nb = min(kStripLanes, pe - bb);
+        // stage: lane q evaluates point bb + q
+        if (lane < nb) {
+            const int p = bb + lane;
+            const int64_t P = pts.idx[p];
+            const unsigned key = pts.key[p];
+            const int64_t j = P - pts.idx_off;
+            const T cr = c[2 * j], ci = c[2 * j + 1];
+            const Pair<T> tt = reinterpret_cast<const Pair<T> *>(pts.rec)[P];
+            T kx[NS], ky[NS];
+            if (horner) {
+                eval_kernel<T, NS, true>(coef, beta, tt.x, kx);
+                eval_kernel<T, NS, true>(coef, beta, tt.y, ky);
+            } else {
+                eval_kernel<T, NS, false>(coef, beta, tt.x, kx);
+                eval_kernel<T, NS, false>(coef, beta, tt.y, ky);
+            }
+#pragma unroll
+            for (int a = 0; a < NS; ++a) lds.kx[lane * NS + a] = kx[a];
+#pragma unroll
+            for (int b = 0; b < NS; ++b) {
+                lds.cky[(lane * NS + b) * 2] = cr * ky[b];
+                lds.cky[(lane * NS + b) * 2 + 1] = ci * ky[b];
+            }
+            lds.lx[lane] = (int)(key & ((1u << kStripKeyShift) - 1));
+            lds.ly[lane] = (int)((key >> kStripKeyShift) - rowbase);
+        }
+        __syncthreads();
+        for (int q = 0; q < nb; ++q) {
+            const int ly = lds.ly[q];
+            while (rcur < ly) {
+                flush(rcur);
+                ++rcur;
+            }
+            const int kk = lane - lds.lx[q];
+            const bool in = kk >= 0 && kk < NS;
+            const T kxv = in ? lds.kx[q * NS + (in ? kk : 0)] : T(0);
+            const T *cy = &lds.cky[q * NS * 2];
+#pragma unroll
+            for (int b = 0; b < NS; ++b) {
+                acc_r[b] = fma(cy[2 * b], kxv, acc_r[b]);
+                acc_i[b] = fma(cy[2 * b + 1], kxv, acc_i[b]);
+            }
+        }
+        __syncthreads();
+    }
+    // drain the window (and, for a full unit, every remaining row of its tile)
+    const int rstop = heavy ? rcur + NS : r0 + rows_tile;
+    while (rcur < rstop) {
+        flush(rcur);
+        ++rcur;
+    }
+}
+
+// ------------------------------------------------------------ ns <= 8 body
+//
+// Lane (a, r) = (lane & 7, lane >> 3) holds the 8 cells of one row whose
+// columns are a, a+8, ..., a+56 of the strip (accumulator k is column 8k + a),
+// for the window row congruent to r mod 8.  An ns x ns footprint then puts
+// exactly one cell in every lane (ns = 8) and each lane adds one complex
+// product per point instead of ns.
+//
+// Which of its eight accumulators a lane adds to depends on the point's
+// 8-column block bx = X >> 3 (X = first tap within the strip) and on whether
+// the lane's column a lies before the point's phase X & 7 (then it is the next
+// block).  The first is made wave-uniform by sorting the points by (row, bx)
+// (sort key low bits) and walking them in runs of equal (row, bx) under a
+// switch; the second by staging, per point and lane column, the weight split
+// into a pair (weight in block bx, weight in block bx + 1), one of them zero.
 
 template <typename T, int BX>
 __device__ __forceinline__ void strip8_walk(const Pair<T> *__restrict__ s_w,
@@ -202,82 +229,70 @@ __device__ __forceinline__ void strip8_walk(const Pair<T> *__restrict__ s_w,
 }
 
 template <typename T, int NS>
-__global__ void __launch_bounds__(kStripLanes)
-    spread_strip8_kernel(StripGeom sg, SortedPts<T> pts, const int *__restrict__ bin_start,
-                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
-                         const T *__restrict__ coef, T beta, int horner, const T *__restrict__ c,
-                         int64_t c_stride, T *__restrict__ fw, int64_t fw_stride,
-                         T *__restrict__ scratch, int64_t scratch_stride) {
+__device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts<T> &pts,
+                                             const int *__restrict__ bin_start, int u, int sub,
+                                             bool heavy, const T *__restrict__ coef, T beta,
+                                             int horner, const T *__restrict__ c,
+                                             T *__restrict__ fw, T *__restrict__ scratch,
+                                             StripLds<T, NS> &lds) {
     static_assert(NS <= 8, "modular ownership needs ns <= 8");
-    // per staged point q: s_w[q*8 + a] = weight pair of lane column a,
-    // s_cky[q*8 + dy] = c * (y-weight of row offset dy), zero for dy >= NS
-    __shared__ Pair<T> s_w[kStripLanes * 8];
-    __shared__ Pair<T> s_cky[kStripLanes * 8];
-    c += 2 * c_stride * blockIdx.y;
-    fw += 2 * fw_stride * blockIdx.y;
-    scratch += 2 * scratch_stride * blockIdx.y;
+    Pair<T> *s_w = lds.w;
+    Pair<T> *s_cky = lds.cky;
     const int lane = threadIdx.x;
     const int a = lane & 7, r = lane >> 3;
-    const int total = *npieces;
     const int rows_tile = sg.H + NS - 1;
-    for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
-        const int u = piece_unit(piece_start, sg.nunits, piece);
-        const int sub = piece - piece_start[u];
-        const bool heavy = piece_start[u + 1] - piece_start[u] > 1;
-        const int s = u / sg.K, k = u - s * sg.K;
-        const int ox = s * sg.SW;
-        const int r0 = k * sg.H;
-        const int r1 = min(r0 + sg.H, (int)sg.nf1);
-        const int ub = bin_start[(int64_t)s * sg.nf1 + r0];
-        const int ue = bin_start[(int64_t)s * sg.nf1 + r1];
-        const int pb = heavy ? ub + sub * sg.pmax : ub;
-        const int pe = heavy ? min(pb + sg.pmax, ue) : ue;
-        T acc_r[8], acc_i[8];
+    const PieceRange pr = piece_range(sg, bin_start, u, sub, heavy);
+    const int ox = pr.ox, r0 = pr.r0, pb = pr.pb, pe = pr.pe;
+    // key >> kStripKeyShift = rowbase + l0y for every point of this strip
+    const int64_t rowbase = pts.bin_off + (int64_t)(u / sg.K) * sg.nf1;
+    T acc_r[8], acc_i[8];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
-        // the window holds rows R .. R+7; row y sits in the lanes with r == y & 7
-        int R = heavy ? (pe > pb ? pts.l0[1][pb] : r0) : r0;
-        T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
-        auto flush = [&](int row) {
-            if ((row & 7) == r) {
-                if (heavy) {
-                    const int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
+    for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
+    // the window holds rows R .. R+7; row y sits in the lanes with r == y & 7
+    int R = heavy ? (pe > pb ? (int)((pts.key[pb] >> kStripKeyShift) - rowbase) : r0) : r0;
+    T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
+    auto flush = [&](int row) {
+        if ((row & 7) == r) {
+            if (heavy) {
+                const int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
 #pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        int64_t x = ox + 8 * b + a;
-                        if (x >= sg.nf0) x -= sg.nf0;
-                        const int64_t gi = y * sg.nf0 + x;
-                        if (acc_r[b] != T(0) || acc_i[b] != T(0)) {
-                            atomic_add_global(&fw[2 * gi], acc_r[b]);
-                            atomic_add_global(&fw[2 * gi + 1], acc_i[b]);
-                        }
+                for (int b = 0; b < 8; ++b) {
+                    int64_t x = ox + 8 * b + a;
+                    if (x >= sg.nf0) x -= sg.nf0;
+                    const int64_t gi = y * sg.nf0 + x;
+                    if (acc_r[b] != T(0) || acc_i[b] != T(0)) {
+                        atomic_add_global(&fw[2 * gi], acc_r[b]);
+                        atomic_add_global(&fw[2 * gi + 1], acc_i[b]);
                     }
-                } else {
-                    Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile) +
-                                   (int64_t)(row - r0) * kStripLanes + a;
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) dst[8 * b] = Pair<T>{acc_r[b], acc_i[b]};
                 }
+            } else {
+                Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile) +
+                               (int64_t)(row - r0) * kStripLanes + a;
 #pragma unroll
-                for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
+                for (int b = 0; b < 8; ++b) dst[8 * b] = Pair<T>{acc_r[b], acc_i[b]};
             }
-        };
-        for (int bb = pb; bb < pe; bb += kStripLanes) {
-            const int nb = min(kStripLanes, pe - bb);
-            int key = -1;  // (row << 3) | block of the lane's staged point
-            if (lane < nb) {
-                const int p = bb + lane;
-                const int64_t j = pts.idx[p];
-                const T cr = c[2 * j], ci = c[2 * j + 1];
-                T kx[NS], ky[NS];
-                if (horner) {
-                    eval_kernel<T, NS, true>(coef, beta, pts.t[0][p], kx);
-                    eval_kernel<T, NS, true>(coef, beta, pts.t[1][p], ky);
-                } else {
-                    eval_kernel<T, NS, false>(coef, beta, pts.t[0][p], kx);
-                    eval_kernel<T, NS, false>(coef, beta, pts.t[1][p], ky);
-                }
-                const int X = pts.l0[0][p] - ox;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
+        }
+    };
+    for (int bb = pb; bb < pe; bb += kStripLanes) {
+        const int nb = min(kStripLanes, pe - bb);
+        int key = -1;  // (row << 3) | block of the lane's staged point
+        if (lane < nb) {
+            const int p = bb + lane;
+            const int64_t P = pts.idx[p];
+            const unsigned pkey = pts.key[p];
+            const int64_t j = P - pts.idx_off;
+            const T cr = c[2 * j], ci = c[2 * j + 1];
+            const Pair<T> tt = reinterpret_cast<const Pair<T> *>(pts.rec)[P];
+            const int X = (int)(pkey & ((1u << kStripKeyShift) - 1));
+            // x-weights, then y-weights: one set of ns values live at a time
+            {
+                T kx[NS];
+                if (horner)
+                    eval_kernel<T, NS, true>(coef, beta, tt.x, kx);
+                else
+                    eval_kernel<T, NS, false>(coef, beta, tt.x, kx);
                 const int ph = X & 7;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
@@ -285,60 +300,76 @@ __global__ void __launch_bounds__(kStripLanes)
                     const bool lo = ph + i < 8;
                     s_w[lane * 8 + ((ph + i) & 7)] = Pair<T>{lo ? v : T(0), lo ? T(0) : v};
                 }
+            }
+            {
+                T ky[NS];
+                if (horner)
+                    eval_kernel<T, NS, true>(coef, beta, tt.y, ky);
+                else
+                    eval_kernel<T, NS, false>(coef, beta, tt.y, ky);
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
                     s_cky[lane * 8 + i] =
                         i < NS ? Pair<T>{cr * ky[i], ci * ky[i]} : Pair<T>{T(0), T(0)};
-                key = (pts.l0[1][p] << 3) | (X >> 3);
             }
-            const int prev = __shfl_up(key, 1);
-            const uint64_t starts = __ballot(lane < nb && (lane == 0 || key != prev));
-            __syncthreads();
-            uint64_t mask = starts;
-            while (mask) {
-                const int q0 = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int q1 = mask ? __builtin_ctzll(mask) : nb;
-                const int gkey = __builtin_amdgcn_readlane(key, q0);
-                const int Y = gkey >> 3;
-                while (R < Y) {
-                    flush(R);
-                    ++R;
-                }
-                const int dy = (r - Y) & 7;
-                switch (gkey & 7) {
-                    case 0: strip8_walk<T, 0>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                    case 1: strip8_walk<T, 1>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                    case 2: strip8_walk<T, 2>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                    case 3: strip8_walk<T, 3>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                    case 4: strip8_walk<T, 4>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                    case 5: strip8_walk<T, 5>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                    case 6: strip8_walk<T, 6>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                    default: strip8_walk<T, 7>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                }
+            key = ((int)((pkey >> kStripKeyShift) - rowbase) << 3) | (X >> 3);
+        }
+        const int prev = __shfl_up(key, 1);
+        const uint64_t starts = __ballot(lane < nb && (lane == 0 || key != prev));
+        __syncthreads();
+        uint64_t mask = starts;
+        while (mask) {
+            const int q0 = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const int q1 = mask ? __builtin_ctzll(mask) : nb;
+            const int gkey = __builtin_amdgcn_readlane(key, q0);
+            const int Y = gkey >> 3;
+            while (R < Y) {
+                flush(R);
+                ++R;
             }
-            __syncthreads();
+            const int dy = (r - Y) & 7;
+            switch (gkey & 7) {
+                case 0: strip8_walk<T, 0>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 1: strip8_walk<T, 1>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 2: strip8_walk<T, 2>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 3: strip8_walk<T, 3>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 4: strip8_walk<T, 4>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 5: strip8_walk<T, 5>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 6: strip8_walk<T, 6>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                default: strip8_walk<T, 7>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+            }
         }
-        // drain the window (and, for a full unit, every remaining row of its tile)
-        const int rstop = heavy ? R + NS : r0 + rows_tile;
-        while (R < rstop) {
-            flush(R);
-            ++R;
-        }
+        __syncthreads();
+    }
+    // drain the window (and, for a full unit, every remaining row of its tile)
+    const int rstop = heavy ? R + NS : r0 + rows_tile;
+    while (R < rstop) {
+        flush(R);
+        ++R;
     }
 }
 
-// fw[y][x] += sum of the scratch tiles of the light units covering (x, y).
 template <typename T, int NS>
-__global__ void __launch_bounds__(256)
-    strip_merge_kernel(StripGeom sg, const int *__restrict__ piece_start,
-                       const T *__restrict__ scratch, int64_t scratch_stride, T *__restrict__ fw,
-                       int64_t fw_stride) {
-    const int64_t F = sg.nf0 * sg.nf1;
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= F) return;
-    fw += 2 * fw_stride * blockIdx.y;
-    scratch += 2 * scratch_stride * blockIdx.y;
+__device__ __forceinline__ void strip_piece(const StripGeom &sg, const StripPts<T> &pts,
+                                            const int *__restrict__ bin_start, int u, int sub,
+                                            bool heavy, const T *__restrict__ coef, T beta,
+                                            int horner, const T *__restrict__ c,
+                                            T *__restrict__ fw, T *__restrict__ scratch,
+                                            StripLds<T, NS> &lds) {
+    if constexpr (NS <= 8)
+        strip8_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, c, fw, scratch,
+                            lds);
+    else
+        wide_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, c, fw, scratch,
+                          lds);
+}
+
+// fw[q] += sum of the scratch tiles of the light units covering cell q.
+template <typename T, int NS>
+__device__ __forceinline__ void merge_cell(const StripGeom &sg, const int *__restrict__ piece_start,
+                                           const T *__restrict__ scratch, T *__restrict__ fw,
+                                           int64_t q) {
     const int x = (int)(q % sg.nf0), y = (int)(q / sg.nf0);
     const int rows_tile = sg.H + NS - 1;
     // candidate strips: the one holding x, the previous one, and the last two
@@ -375,23 +406,25 @@ __global__ void __launch_bounds__(256)
     fw[2 * q + 1] = ai;
 }
 
-// One workgroup: pieces per unit (1, or ceil(count / pmax) for a crowded
-// unit) and their exclusive prefix; total at [nunits] and in *npieces.
-__global__ void __launch_bounds__(1024)
-    strip_units_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
-                       int *npieces) {
+// pieces of unit u: 1, or ceil(count / pmax) for a crowded unit
+__device__ __forceinline__ int unit_pieces(const StripGeom &sg, const int *__restrict__ bin_start,
+                                           int u) {
+    const int s = u / sg.K, k = u - s * sg.K;
+    const int r0 = k * sg.H, r1 = min(r0 + sg.H, (int)sg.nf1);
+    const int cnt = bin_start[(int64_t)s * sg.nf1 + r1] - bin_start[(int64_t)s * sg.nf1 + r0];
+    return cnt <= sg.pmax ? 1 : (cnt + sg.pmax - 1) / sg.pmax;
+}
+
+// One workgroup: exclusive prefix of pieces(u) over u in [0, n); total at
+// piece_start[n] and in *npieces.
+template <typename PiecesOf>
+__device__ __forceinline__ void piece_scan(int n, PiecesOf pieces, int *piece_start,
+                                           int *npieces) {
     __shared__ int sa[1024];
     const int tid = threadIdx.x;
-    const int n = sg.nunits;
     const int per = (n + 1023) / 1024;
-    const int lo = tid * per;
+    const int lo = min(tid * per, n);
     const int hi = min(lo + per, n);
-    auto pieces = [&](int u) {
-        const int s = u / sg.K, k = u - s * sg.K;
-        const int r0 = k * sg.H, r1 = min(r0 + sg.H, (int)sg.nf1);
-        const int cnt = bin_start[(int64_t)s * sg.nf1 + r1] - bin_start[(int64_t)s * sg.nf1 + r0];
-        return cnt <= sg.pmax ? 1 : (cnt + sg.pmax - 1) / sg.pmax;
-    };
     int a = 0;
     for (int u = lo; u < hi; ++u) a += pieces(u);
     sa[tid] = a;
@@ -413,10 +446,115 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
+// ------------------------------------------------------------ one plan
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(kStripLanes)
+    spread_strip_kernel(StripGeom sg, StripPts<T> pts, const int *__restrict__ bin_start,
+                        const int *__restrict__ piece_start, const int *__restrict__ npieces,
+                        const T *__restrict__ coef, T beta, int horner, const T *__restrict__ c,
+                        int64_t c_stride, T *__restrict__ fw, int64_t fw_stride,
+                        T *__restrict__ scratch, int64_t scratch_stride) {
+    __shared__ StripLds<T, NS> lds;
+    c += 2 * c_stride * blockIdx.y;
+    fw += 2 * fw_stride * blockIdx.y;
+    scratch += 2 * scratch_stride * blockIdx.y;
+    const int total = *npieces;
+    for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
+        const int u = segment_of(piece_start, sg.nunits, piece);
+        const int sub = piece - piece_start[u];
+        const bool heavy = piece_start[u + 1] - piece_start[u] > 1;
+        strip_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, c, fw, scratch,
+                           lds);
+    }
+}
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(256)
+    strip_merge_kernel(StripGeom sg, const int *__restrict__ piece_start,
+                       const T *__restrict__ scratch, int64_t scratch_stride, T *__restrict__ fw,
+                       int64_t fw_stride) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= sg.nf0 * sg.nf1) return;
+    merge_cell<T, NS>(sg, piece_start, scratch + 2 * scratch_stride * blockIdx.y,
+                      fw + 2 * fw_stride * blockIdx.y, q);
+}
+
+__global__ void __launch_bounds__(1024)
+    strip_units_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
+                       int *npieces) {
+    piece_scan(
+        sg.nunits, [&](int u) { return unit_pieces(sg, bin_start, u); }, piece_start, npieces);
+}
+
+// ------------------------------------------------------------ ragged batch
+//
+// Virtual problem v = (problem, transform).  Its units are the global units
+// [vunit_off[v], vunit_off[v+1]); its points are its problem's, found through
+// the problem's slice of the global bins; its fine grid and scratch tiles are
+// slices of the concatenated buffers; its strengths come from the pointer
+// table (c of problem i at ptrs[2*nprob + i], transform-major).
+
+__global__ void __launch_bounds__(1024)
+    ragged_units_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ vunit_off,
+                        int nv, const int *__restrict__ bin_start, int *piece_start,
+                        int *npieces) {
+    const int nu = vunit_off[nv];
+    piece_scan(
+        nu,
+        [&](int gu) {
+            const int v = segment_of(vunit_off, nv, gu);
+            const RaggedVirt &d = virt[v];
+            return unit_pieces(d.sg, bin_start + d.bin_off, gu - d.unit_off);
+        },
+        piece_start, npieces);
+}
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(kStripLanes)
+    ragged_strip_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ vunit_off,
+                        int nv, StripPts<T> pts, const int *__restrict__ bin_start,
+                        const int *__restrict__ piece_start, const int *__restrict__ npieces,
+                        const T *__restrict__ coef, T beta, int horner,
+                        const void *const *__restrict__ ptrs, int nprob, T *__restrict__ fw_all,
+                        T *__restrict__ scratch_all) {
+    __shared__ StripLds<T, NS> lds;
+    const int total = *npieces;
+    const int nu = vunit_off[nv];
+    for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
+        const int gu = segment_of(piece_start, nu, piece);
+        const int v = segment_of(vunit_off, nv, gu);
+        const RaggedVirt d = virt[v];
+        const int sub = piece - piece_start[gu];
+        const bool heavy = piece_start[gu + 1] - piece_start[gu] > 1;
+        const T *c = (const T *)ptrs[2 * nprob + d.prob] + 2 * (int64_t)d.t * d.m;
+        StripPts<T> vp = pts;
+        vp.bin_off = d.bin_off;
+        vp.idx_off = d.moff;
+        strip_piece<T, NS>(d.sg, vp, bin_start + d.bin_off, gu - d.unit_off, sub, heavy, coef,
+                           beta, horner, c, fw_all + 2 * d.fw_off, scratch_all + 2 * d.tile_off,
+                           lds);
+    }
+}
+
+// block b merges 256 cells of the virtual problem v with cell_blk[v] <= b
+template <typename T, int NS>
+__global__ void __launch_bounds__(256)
+    ragged_merge_kernel(const RaggedVirt *__restrict__ virt, const int64_t *__restrict__ cell_blk,
+                        int nv, const int *__restrict__ piece_start,
+                        const T *__restrict__ scratch_all, T *__restrict__ fw_all) {
+    const int v = segment_of(cell_blk, nv, (int64_t)blockIdx.x);
+    const RaggedVirt d = virt[v];
+    const int64_t q = ((int64_t)blockIdx.x - cell_blk[v]) * 256 + threadIdx.x;
+    if (q >= d.sg.nf0 * d.sg.nf1) return;
+    merge_cell<T, NS>(d.sg, piece_start + d.unit_off, scratch_all + 2 * d.tile_off,
+                      fw_all + 2 * d.fw_off, q);
+}
+
 }  // namespace
 
 template <typename T>
-int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int *bin_start,
+int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, const T *coef, int kerevalmeth,
                         double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
                         T *scratch, int64_t scratch_stride, int ntr, int phases,
@@ -431,16 +569,10 @@ int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int 
     const T b = (T)beta;
     const int64_t F = sg.nf0 * sg.nf1;
     JFA_DISPATCH_NS(sg.ns, {
-        if (phases & kStripSpread) {
-            if constexpr (NS_ <= 8)
-                spread_strip8_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
-                    sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride,
-                    fw, fw_stride, scratch, scratch_stride);
-            else
-                spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
-                    sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride,
-                    fw, fw_stride, scratch, scratch_stride);
-        }
+        if (phases & kStripSpread)
+            spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
+                sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride, fw,
+                fw_stride, scratch, scratch_stride);
         if (phases & kStripMerge)
             strip_merge_kernel<T, NS_>
                 <<<dim3((unsigned)((F + 255) / 256), ntr), 256, 0, stream>>>(
@@ -450,13 +582,42 @@ int launch_spread_strip(const StripGeom &sg, const SortedPts<T> &pts, const int 
     return JFA_OK;
 }
 
-template int launch_spread_strip<float>(const StripGeom &, const SortedPts<float> &, const int *,
-                                        int *, int *, const float *, int, double, const float *,
-                                        int64_t, float *, int64_t, float *, int64_t, int,
-                                        int, hipStream_t);
-template int launch_spread_strip<double>(const StripGeom &, const SortedPts<double> &,
-                                         const int *, int *, int *, const double *, int, double,
-                                         const double *, int64_t, double *, int64_t, double *,
-                                         int64_t, int, int, hipStream_t);
+template <typename T>
+int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T *coef,
+                        int kerevalmeth, double beta, T *fw_all, T *scratch_all, int phases,
+                        hipStream_t stream) {
+    // no points anywhere: the zeroed fine grids are already the result
+    if (rl.nv == 0 || pts.M == 0) return JFA_OK;
+    if (phases & kStripUnits) {
+        ragged_units_kernel<<<1, 1024, 0, stream>>>(rl.virt, rl.vunit_off, rl.nv, rl.bin_start,
+                                                    rl.piece_start, rl.npieces);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    const int64_t ub = rl.nunits + pts.M / rl.pmax + rl.nv;
+    const unsigned grid = (unsigned)(ub < 256 * 32 ? ub : 256 * 32);
+    const T b = (T)beta;
+    JFA_DISPATCH_NS(rl.ns, {
+        if (phases & kStripSpread)
+            ragged_strip_kernel<T, NS_><<<grid, kStripLanes, 0, stream>>>(
+                rl.virt, rl.vunit_off, rl.nv, pts, rl.bin_start, rl.piece_start, rl.npieces, coef,
+                b, kerevalmeth, rl.ptrs, rl.nprob, fw_all, scratch_all);
+        if (phases & kStripMerge)
+            ragged_merge_kernel<T, NS_><<<(unsigned)rl.cell_blocks, 256, 0, stream>>>(
+                rl.virt, rl.cell_blk, rl.nv, rl.piece_start, scratch_all, fw_all);
+    });
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+#define JFA_INST(T)                                                                           \
+    template int launch_spread_strip<T>(const StripGeom &, const StripPts<T> &, const int *, \
+                                        int *, int *, const T *, int, double, const T *,      \
+                                        int64_t, T *, int64_t, T *, int64_t, int, int,        \
+                                        hipStream_t);                                         \
+    template int launch_ragged_strip<T>(const RaggedLaunch &, const StripPts<T> &, const T *, \
+                                        int, double, T *, T *, int, hipStream_t);
+JFA_INST(float)
+JFA_INST(double)
+#undef JFA_INST
 
 }  // namespace jfa

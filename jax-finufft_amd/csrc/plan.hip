@@ -25,11 +25,11 @@
 using namespace jfa;
 
 namespace {
-
 std::once_flag g_rocfft_once;
 int g_rocfft_status = 0;
+}  // namespace
 
-int ensure_rocfft() {
+int jfa::ensure_rocfft() {
     std::call_once(g_rocfft_once, [] {
         g_rocfft_status = rocfft_setup() == rocfft_status_success ? 0 : 1;
     });
@@ -39,6 +39,8 @@ int ensure_rocfft() {
     }
     return JFA_OK;
 }
+
+namespace {
 
 template <typename T>
 int upload(const std::vector<double> &h, void **d) {
@@ -151,15 +153,17 @@ struct DeviceGuard {
 
 bool jfa::strip_applicable(jfa_plan p) {
     const GridGeom &g = p->geom;
-    return p->dim == 2 && g.nf[0] >= 64 && g.nf[1] >= 2 * g.ns;
+    if (p->dim != 2 || g.nf[0] < 64 || g.nf[1] < 2 * g.ns) return false;
+    // (strip, row) bins shifted by the column bits must fit 32-bit sort keys
+    const int64_t strips = (g.nf[0] + 64 - g.ns) / (64 - g.ns + 1);
+    return ((strips * g.nf[1] + 1) << kStripKeyShift) <= (int64_t)UINT32_MAX;
 }
 
 // Strip geometry and bins for the 2D strip spreader, and its buffers.  Units
 // of H rows are sized for about 4096 units (a few waves per SIMD), with H in
 // [ns, nf1 - ns + 1] so that a cell is covered by at most two units per axis.
-int jfa::setup_strip(jfa_plan p) {
-    GridGeom &g = p->geom;
-    StripGeom &sg = p->sg;
+jfa::StripGeom jfa::strip_geometry(GridGeom &g, int pmax) {
+    StripGeom sg{};
     const int ns = g.ns;
     sg.nf0 = g.nf[0];
     sg.nf1 = g.nf[1];
@@ -173,7 +177,7 @@ int jfa::setup_strip(jfa_plan p) {
     sg.H = (int)H;
     sg.K = (int)((g.nf[1] + H - 1) / H);
     sg.nunits = sg.S * sg.K;
-    sg.pmax = p->opts.gpu_maxsubprobsize > 0 ? p->opts.gpu_maxsubprobsize : 4096;
+    sg.pmax = pmax > 0 ? pmax : 4096;
     // bins: (strip, exact row), rows fastest
     g.bs[0] = sg.SW;
     g.bs[1] = 1;
@@ -183,8 +187,16 @@ int jfa::setup_strip(jfa_plan p) {
     g.nb[2] = 1;
     g.nbins = (int64_t)g.nb[0] * g.nb[1];
     g.ymajor = 1;
-    // the ns <= 8 spreader also wants each row's points grouped by 8-column block
-    g.keyshift = ns <= 8 ? 3 : 0;
+    // sort keys also carry the first tap's column within the strip
+    g.keyshift = kStripKeyShift;
+    return sg;
+}
+
+int jfa::setup_strip(jfa_plan p) {
+    GridGeom &g = p->geom;
+    StripGeom &sg = p->sg;
+    const int ns = g.ns;
+    sg = strip_geometry(g, p->opts.gpu_maxsubprobsize);
     // buffers
     free_ptr(reinterpret_cast<void *&>(p->d_piece_start));
     JFA_HIP_CHECK(hipMalloc(&p->d_piece_start, sizeof(int) * (sg.nunits + 1)));
@@ -561,8 +573,13 @@ int jfa::reserve_points(jfa_plan p, int64_t M) {
 template <typename T>
 static int sort_points(jfa_plan p, int64_t M, const T *x, const T *y, const T *z) {
     const GridGeom &g = p->geom;
-    const bool sorted =
-        p->method == JFA_METHOD_SM || p->method == JFA_METHOD_STRIP || p->opts.gpu_sort;
+    if (p->method == JFA_METHOD_STRIP) {
+        SortScratch s{p->d_key_in, p->d_val_in, p->d_key_out, p->d_sort_temp,
+                      p->sort_temp_bytes};
+        return launch_sort_points_strip<T>(g, M, x, y, s, p->d_bin_start, p->d_idx,
+                                           (T *)p->d_t, p->stream);
+    }
+    const bool sorted = p->method == JFA_METHOD_SM || p->opts.gpu_sort;
     if (!sorted)
         return launch_fold_points<T>(g, M, x, y, z, p->d_idx, p->d_l0, (T *)p->d_t, p->stream);
     SortScratch s{p->d_key_in, p->d_val_in, p->d_key_out, p->d_sort_temp, p->sort_temp_bytes};
@@ -639,14 +656,15 @@ BinIndex jfa::bin_view(jfa_plan p) {
 template <typename T>
 static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
                         int ntr, StageTimer *tm) {
-    const SortedPts<T> pts = sorted_view<T>(p);
     const int kem = p->opts.gpu_kerevalmeth ? 1 : 0;
     if (p->method != JFA_METHOD_STRIP) {
-        int rc = launch_spread<T>(p->method, p->geom, pts, bin_view(p), (const T *)p->d_coef, kem,
-                                  p->kp.beta, c, c_stride, fw, fw_stride, ntr, p->stream);
+        int rc = launch_spread<T>(p->method, p->geom, sorted_view<T>(p), bin_view(p),
+                                  (const T *)p->d_coef, kem, p->kp.beta, c, c_stride, fw,
+                                  fw_stride, ntr, p->stream);
         if (tm) tm->mark("spread");
         return rc;
     }
+    const StripPts<T> pts{p->M, p->d_idx, p->d_key_out, (const T *)p->d_t, 0, 0};
     for (int t0 = 0; t0 < ntr; t0 += p->batch) {
         const int nb = std::min(p->batch, ntr - t0);
         const int phases[3] = {kStripUnits, kStripSpread, kStripMerge};

@@ -1,0 +1,568 @@
+// Ragged batches of 2D type-1 problems: one launch sequence for many
+// independent problems with their own point counts, mode grids and numbers of
+// stacked strength vectors.  No reference counterpart: with the reference a
+// batch like this is a Python loop of nufft1 calls, each making, using and
+// destroying its own plan (run_nufft, lib/kernels.cc.cu:25-92).
+//
+// Layout.  The points of all problems form one concatenated index space
+// (problem i owns [moff[i], moff[i+1])) and are sorted together: a point's key
+// is its bin within its problem offset by the problem's first global bin.  A
+// virtual problem is one (problem, transform) pair; virtual problems are
+// ordered by fine-grid size so that the fine grids of one size are contiguous
+// and transform together in one batched rocFFT call.  The spreading kernel
+// walks the global list of units of every virtual problem; merge and
+// deconvolution walk the global lists of 256-cell / 256-mode blocks.  The
+// device pointer table (x, y, c, f of every problem) is rewritten before each
+// run, so a captured graph stays valid for new arrays of the same shapes.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "jfa_internal.hpp"
+
+using namespace jfa;
+
+struct jfa_ragged_s {
+    int dim = 2, nprob = 0, is_double = 0, iflag = 1, device = 0;
+    double eps = 1e-6;
+    jfa_opts opts{};
+    KernelParams kp;
+    std::vector<int64_t> npts, ntr, n0, n1;
+    std::vector<RaggedProb> probs;
+    std::vector<RaggedVirt> virt;
+    std::vector<int64_t> moff, cell_blk, mode_blk;
+    std::vector<int> vunit_off;
+    RaggedLaunch rl{};
+
+    // device tables
+    void *d_probs = nullptr, *d_virt = nullptr, *d_moff = nullptr, *d_vunit_off = nullptr,
+         *d_cell_blk = nullptr, *d_mode_blk = nullptr;
+    void *d_ptrs = nullptr;
+    const void **h_ptrs = nullptr;  // pinned staging of the pointer table
+    hipEvent_t ptrs_copied = nullptr;
+    void *d_coef = nullptr, *d_dk = nullptr;
+
+    // sorted points
+    int64_t mtot = 0;
+    int *d_idx = nullptr, *d_l0 = nullptr;
+    void *d_t = nullptr;
+    unsigned *d_key_in = nullptr, *d_val_in = nullptr, *d_key_out = nullptr;
+    void *d_sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    int *d_bin_start = nullptr, *d_piece_start = nullptr, *d_npieces = nullptr;
+
+    // fine grids and strip scratch (complex cells)
+    int64_t fw_cells = 0, tile_cells = 0;
+    void *d_fw = nullptr, *d_scratch = nullptr;
+
+    // one batched rocFFT per fine-grid size
+    struct FftClass {
+        int64_t nf0, nf1, fw_off;
+        int count;
+        rocfft_plan plan;
+    };
+    std::vector<FftClass> fft;
+    rocfft_execution_info fft_info = nullptr;
+    void *d_fft_work = nullptr;
+    size_t fft_work_bytes = 0;
+
+    // streams and graph
+    hipStream_t stream = nullptr;  // the caller's stream
+    hipStream_t work = nullptr;    // own stream that graphs are captured on
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    bool use_graph = false;
+    int graph_state = 0;           // 0 not captured yet, 1 captured, -1 capture failed
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+void free_dev(void *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename V>
+int upload_vec(const std::vector<V> &h, void **d) {
+    JFA_HIP_CHECK(hipMalloc(d, sizeof(V) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty())
+        JFA_HIP_CHECK(hipMemcpy(*d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
+    return JFA_OK;
+}
+
+template <typename T>
+int upload_real(const std::vector<double> &h, void **d) {
+    std::vector<T> tmp(h.begin(), h.end());
+    return upload_vec(tmp, d);
+}
+
+void release(jfa_ragged r) {
+    for (void **q : {&r->d_probs, &r->d_virt, &r->d_moff, &r->d_vunit_off, &r->d_cell_blk,
+                     &r->d_mode_blk, &r->d_ptrs, &r->d_coef, &r->d_dk, (void **)&r->d_idx,
+                     (void **)&r->d_l0, &r->d_t, (void **)&r->d_key_in, (void **)&r->d_val_in,
+                     (void **)&r->d_key_out, &r->d_sort_temp, (void **)&r->d_bin_start,
+                     (void **)&r->d_piece_start, (void **)&r->d_npieces, &r->d_fw, &r->d_scratch,
+                     &r->d_fft_work})
+        free_dev(*q);
+    if (r->h_ptrs) (void)hipHostFree((void *)r->h_ptrs);
+    r->h_ptrs = nullptr;
+    for (auto &c : r->fft)
+        if (c.plan) rocfft_plan_destroy(c.plan);
+    r->fft.clear();
+    if (r->fft_info) rocfft_execution_info_destroy(r->fft_info);
+    r->fft_info = nullptr;
+    if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+    r->gexec = nullptr;
+    for (hipEvent_t *e : {&r->ev_in, &r->ev_out, &r->ptrs_copied})
+        if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+    if (r->work) (void)hipStreamDestroy(r->work);
+    r->work = nullptr;
+}
+
+struct DeviceScope {
+    int prev = -1;
+    bool changed = false;
+    int set(int dev) {
+        JFA_HIP_CHECK(hipGetDevice(&prev));
+        if (prev != dev) {
+            JFA_HIP_CHECK(hipSetDevice(dev));
+            changed = true;
+        }
+        return JFA_OK;
+    }
+    ~DeviceScope() {
+        if (changed) (void)hipSetDevice(prev);
+    }
+};
+
+// Problem geometry, virtual problems, offsets; host only.
+int layout(jfa_ragged r) {
+    const int ns = r->kp.ns;
+    const int np = r->nprob;
+    r->probs.assign(np, RaggedProb{});
+    r->moff.assign(np + 1, 0);
+    std::vector<StripGeom> sg(np);
+    int64_t bins = 0;
+    for (int i = 0; i < np; ++i) {
+        GridGeom &g = r->probs[i].g;
+        g.dim = 2;
+        g.ns = ns;
+        const int64_t n[2] = {r->n0[i], r->n1[i]};
+        for (int d = 0; d < 3; ++d) {
+            g.nf[d] = d < 2 ? fine_size(n[d], r->opts.upsampfac, ns) : 1;
+            g.scale[d] = d < 2 ? (double)g.nf[d] / (2.0 * M_PI) : 1.0;
+            g.shift[d] = 0.0;
+        }
+        if (g.nf[0] < 64 || g.nf[1] < 2 * ns) {
+            set_error("ragged problem %d: fine grid %lld x %lld is below the strip spreader's "
+                      "64 x %d minimum", i, (long long)g.nf[0], (long long)g.nf[1], 2 * ns);
+            return JFA_ERR_NOT_SUPPORTED;
+        }
+        if (g.nf[0] * g.nf[1] > kMaxFineCells) {
+            set_error("ragged problem %d: fine grid exceeds %lld cells", i,
+                      (long long)kMaxFineCells);
+            return JFA_ERR_TOO_LARGE;
+        }
+        sg[i] = strip_geometry(g, r->opts.gpu_maxsubprobsize);
+        r->probs[i].moff = r->moff[i];
+        r->probs[i].bin_off = bins;
+        r->moff[i + 1] = r->moff[i] + r->npts[i];
+        bins += g.nbins;
+    }
+    r->mtot = r->moff[np];
+    if (r->mtot > INT_MAX) {
+        set_error("ragged batch of %lld points exceeds 2^31 - 1", (long long)r->mtot);
+        return JFA_ERR_TOO_LARGE;
+    }
+    const int keyshift = r->probs.empty() ? 0 : r->probs[0].g.keyshift;
+    if (((bins + 1) << keyshift) > (int64_t)UINT32_MAX) {
+        set_error("ragged batch has too many bins for 32-bit sort keys");
+        return JFA_ERR_TOO_LARGE;
+    }
+    // virtual problems, grouped by fine-grid size
+    std::vector<int> order(np);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const GridGeom &ga = r->probs[a].g, &gb = r->probs[b].g;
+        return ga.nf[0] != gb.nf[0] ? ga.nf[0] < gb.nf[0] : ga.nf[1] < gb.nf[1];
+    });
+    std::vector<int64_t> dk_off(np);
+    int64_t dk = 0;
+    for (int i = 0; i < np; ++i) {
+        dk_off[i] = dk;
+        dk += r->n0[i] + r->n1[i];
+    }
+    r->virt.clear();
+    r->vunit_off.assign(1, 0);
+    r->cell_blk.assign(1, 0);
+    r->mode_blk.assign(1, 0);
+    r->fft.clear();
+    int64_t fw = 0, tiles = 0;
+    for (int i : order) {
+        const GridGeom &g = r->probs[i].g;
+        const int64_t F = g.nf[0] * g.nf[1];
+        if (r->fft.empty() || r->fft.back().nf0 != g.nf[0] || r->fft.back().nf1 != g.nf[1])
+            r->fft.push_back({g.nf[0], g.nf[1], fw, 0, nullptr});
+        for (int t = 0; t < r->ntr[i]; ++t) {
+            RaggedVirt v{};
+            v.sg = sg[i];
+            v.prob = i;
+            v.t = t;
+            v.m = r->npts[i];
+            v.bin_off = r->probs[i].bin_off;
+            v.fw_off = fw;
+            v.tile_off = tiles;
+            v.unit_off = r->vunit_off.back();
+            v.n0 = r->n0[i];
+            v.n1 = r->n1[i];
+            v.dk_off = dk_off[i];
+            v.moff = r->moff[i];
+            r->virt.push_back(v);
+            fw += F;
+            tiles += (int64_t)sg[i].nunits * (sg[i].H + ns - 1) * 64;
+            r->vunit_off.push_back(r->vunit_off.back() + sg[i].nunits);
+            r->cell_blk.push_back(r->cell_blk.back() + (F + 255) / 256);
+            r->mode_blk.push_back(r->mode_blk.back() + (v.n0 * v.n1 + 255) / 256);
+            r->fft.back().count++;
+        }
+    }
+    r->fw_cells = fw;
+    r->tile_cells = tiles;
+    if ((int64_t)r->vunit_off.back() > INT_MAX / 2) {
+        set_error("ragged batch has too many spreading units");
+        return JFA_ERR_TOO_LARGE;
+    }
+
+    RaggedLaunch &rl = r->rl;
+    rl.nprob = np;
+    rl.nv = (int)r->virt.size();
+    rl.ns = ns;
+    rl.pmax = sg.empty() ? 4096 : sg[0].pmax;
+    rl.keyshift = keyshift;
+    rl.mtot = r->mtot;
+    rl.nbins = bins;
+    rl.nunits = r->vunit_off.back();
+    rl.cell_blocks = r->cell_blk.back();
+    rl.mode_blocks = r->mode_blk.back();
+    return JFA_OK;
+}
+
+int allocate(jfa_ragged r) {
+    const size_t real = r->is_double ? 8 : 4;
+    const int np = r->nprob;
+    if (int rc = upload_vec(r->probs, &r->d_probs)) return rc;
+    if (int rc = upload_vec(r->virt, &r->d_virt)) return rc;
+    if (int rc = upload_vec(r->moff, &r->d_moff)) return rc;
+    if (int rc = upload_vec(r->vunit_off, &r->d_vunit_off)) return rc;
+    if (int rc = upload_vec(r->cell_blk, &r->d_cell_blk)) return rc;
+    if (int rc = upload_vec(r->mode_blk, &r->d_mode_blk)) return rc;
+    JFA_HIP_CHECK(hipMalloc(&r->d_ptrs, sizeof(void *) * 4 * np));
+    JFA_HIP_CHECK(hipHostMalloc((void **)&r->h_ptrs, sizeof(void *) * 4 * np, 0));
+    JFA_HIP_CHECK(hipEventCreateWithFlags(&r->ptrs_copied, hipEventDisableTiming));
+
+    // kernel tables: Horner coefficients, and 1 / psi_hat per problem and axis
+    const KernelParams &kp = r->kp;
+    std::vector<double> coef((size_t)kp.nc * kp.ns);
+    horner_fit(kp.ns, kp.nc, kp.beta, coef.data());
+    std::vector<double> dk;
+    for (int i = 0; i < np; ++i) {
+        for (int d = 0; d < 2; ++d) {
+            const int64_t n = d == 0 ? r->n0[i] : r->n1[i];
+            std::vector<double> ph((size_t)n);
+            kernel_fourier(kp.ns, kp.beta, (double)r->probs[i].g.nf[d], (double)(-(n / 2)), n,
+                           ph.data());
+            for (double v : ph) dk.push_back(1.0 / v);
+        }
+    }
+    int rc = r->is_double ? upload_real<double>(coef, &r->d_coef)
+                          : upload_real<float>(coef, &r->d_coef);
+    if (rc) return rc;
+    rc = r->is_double ? upload_real<double>(dk, &r->d_dk) : upload_real<float>(dk, &r->d_dk);
+    if (rc) return rc;
+
+    // sort scratch
+    const int64_t cap = std::max<int64_t>(r->mtot, 1);
+    JFA_HIP_CHECK(hipMalloc(&r->d_idx, sizeof(int) * cap));
+    JFA_HIP_CHECK(hipMalloc(&r->d_t, real * cap * 2));
+    JFA_HIP_CHECK(hipMalloc(&r->d_key_in, sizeof(unsigned) * cap));
+    JFA_HIP_CHECK(hipMalloc(&r->d_val_in, sizeof(unsigned) * cap));
+    JFA_HIP_CHECK(hipMalloc(&r->d_key_out, sizeof(unsigned) * cap));
+    r->sort_temp_bytes = sort_temp_bytes(cap);
+    if (!r->sort_temp_bytes) {
+        set_error("radix sort temp size query failed");
+        return JFA_ERR_HIP;
+    }
+    JFA_HIP_CHECK(hipMalloc(&r->d_sort_temp, r->sort_temp_bytes));
+    JFA_HIP_CHECK(hipMalloc(&r->d_bin_start, sizeof(int) * (r->rl.nbins + 1)));
+    JFA_HIP_CHECK(hipMalloc(&r->d_piece_start, sizeof(int) * (r->rl.nunits + 1)));
+    JFA_HIP_CHECK(hipMalloc(&r->d_npieces, sizeof(int) * 4));
+    if (hipMalloc(&r->d_fw, 2 * real * std::max<int64_t>(r->fw_cells, 1)) != hipSuccess ||
+        hipMalloc(&r->d_scratch, 2 * real * std::max<int64_t>(r->tile_cells, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("could not allocate the ragged batch's fine grids (%lld cells) and tiles",
+                  (long long)r->fw_cells);
+        return JFA_ERR_ALLOC;
+    }
+
+    // one batched in-place FFT per fine-grid size
+    for (auto &c : r->fft) {
+        size_t lengths[2] = {(size_t)c.nf0, (size_t)c.nf1};
+        rocfft_status st = rocfft_plan_create(
+            &c.plan, rocfft_placement_inplace,
+            r->iflag > 0 ? rocfft_transform_type_complex_inverse
+                         : rocfft_transform_type_complex_forward,
+            r->is_double ? rocfft_precision_double : rocfft_precision_single, 2, lengths,
+            (size_t)c.count, nullptr);
+        if (st != rocfft_status_success) {
+            c.plan = nullptr;
+            set_error("rocfft_plan_create failed (status %d)", (int)st);
+            return JFA_ERR_FFT;
+        }
+        size_t ws = 0;
+        rocfft_plan_get_work_buffer_size(c.plan, &ws);
+        r->fft_work_bytes = std::max(r->fft_work_bytes, ws);
+    }
+    if (r->fft_work_bytes) JFA_HIP_CHECK(hipMalloc(&r->d_fft_work, r->fft_work_bytes));
+    if (rocfft_execution_info_create(&r->fft_info) != rocfft_status_success) {
+        set_error("rocfft_execution_info_create failed");
+        return JFA_ERR_FFT;
+    }
+    if (r->fft_work_bytes)
+        rocfft_execution_info_set_work_buffer(r->fft_info, r->d_fft_work, r->fft_work_bytes);
+
+    if (r->use_graph) {
+        JFA_HIP_CHECK(hipStreamCreateWithFlags(&r->work, hipStreamNonBlocking));
+        JFA_HIP_CHECK(hipEventCreateWithFlags(&r->ev_in, hipEventDisableTiming));
+        JFA_HIP_CHECK(hipEventCreateWithFlags(&r->ev_out, hipEventDisableTiming));
+    }
+
+    RaggedLaunch &rl = r->rl;
+    rl.probs = (const RaggedProb *)r->d_probs;
+    rl.virt = (const RaggedVirt *)r->d_virt;
+    rl.moff = (const int64_t *)r->d_moff;
+    rl.vunit_off = (const int *)r->d_vunit_off;
+    rl.cell_blk = (const int64_t *)r->d_cell_blk;
+    rl.mode_blk = (const int64_t *)r->d_mode_blk;
+    rl.ptrs = (const void *const *)r->d_ptrs;
+    rl.bin_start = r->d_bin_start;
+    rl.piece_start = r->d_piece_start;
+    rl.npieces = r->d_npieces;
+    return JFA_OK;
+}
+
+// sorted keys and concatenated-order indices; Horner variables by index
+template <typename T>
+StripPts<T> sorted(jfa_ragged r) {
+    return StripPts<T>{r->mtot, r->d_idx, r->d_key_out, (const T *)r->d_t, 0, 0};
+}
+
+// Everything after the sort: zero the fine grids, spread, merge, FFT, deconvolve.
+template <typename T>
+int enqueue_execute(jfa_ragged r, hipStream_t s) {
+    const int kem = r->opts.gpu_kerevalmeth ? 1 : 0;
+    if (int rc = launch_fill_zero(r->d_fw, (int64_t)(2 * sizeof(T)) * r->fw_cells, s)) return rc;
+    if (int rc = launch_ragged_strip<T>(r->rl, sorted<T>(r), (const T *)r->d_coef, kem,
+                                        r->kp.beta, (T *)r->d_fw, (T *)r->d_scratch, kStripAll,
+                                        s))
+        return rc;
+    rocfft_execution_info_set_stream(r->fft_info, s);
+    for (auto &c : r->fft) {
+        void *bufs[1] = {(T *)r->d_fw + 2 * c.fw_off};
+        rocfft_status st = rocfft_execute(c.plan, bufs, nullptr, r->fft_info);
+        if (st != rocfft_status_success) {
+            set_error("rocfft_execute failed (status %d)", (int)st);
+            return JFA_ERR_FFT;
+        }
+    }
+    return launch_ragged_deconv<T>(r->rl, (const T *)r->d_dk, r->opts.modeord,
+                                   (const T *)r->d_fw, s);
+}
+
+template <typename T>
+int run_t(jfa_ragged r, hipStream_t s) {
+    SortScratch ss{r->d_key_in, r->d_val_in, r->d_key_out, r->d_sort_temp, r->sort_temp_bytes};
+    if (int rc = launch_ragged_sort<T>(r->rl, ss, r->d_idx, (T *)r->d_t, s)) return rc;
+    if (!r->use_graph || r->graph_state < 0) return enqueue_execute<T>(r, s);
+    if (r->graph_state == 0) {
+        // capture once; a library call that cannot be captured ends graph use
+        // for this plan and the run proceeds eagerly
+        hipGraph_t graph = nullptr;
+        int rc = JFA_OK;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            r->graph_state = -1;
+            return enqueue_execute<T>(r, s);
+        }
+        rc = enqueue_execute<T>(r, s);
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc != JFA_OK || e != hipSuccess || !graph) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            r->graph_state = -1;
+            return enqueue_execute<T>(r, s);
+        }
+        e = hipGraphInstantiate(&r->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            r->gexec = nullptr;
+            r->graph_state = -1;
+            return enqueue_execute<T>(r, s);
+        }
+        r->graph_state = 1;
+    }
+    JFA_HIP_CHECK(hipGraphLaunch(r->gexec, s));
+    return JFA_OK;
+}
+
+}  // namespace
+
+extern "C" int jfa_ragged_makeplan(int dim, int nprob, const int64_t *nmodes,
+                                   const int64_t *ntransf, const int64_t *npoints, int iflag,
+                                   double eps, int is_double, const jfa_opts *opts, int use_graph,
+                                   jfa_ragged *plan_out) {
+    if (!plan_out) {
+        set_error("plan_out is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    *plan_out = nullptr;
+    if (dim != 2) {
+        set_error("ragged batches are 2D type-1 problems (dim %d requested)", dim);
+        return JFA_ERR_NOT_SUPPORTED;
+    }
+    if (nprob < 1 || !nmodes || !ntransf || !npoints) {
+        set_error("ragged batch needs nprob >= 1 and the nmodes / ntransf / npoints arrays");
+        return JFA_ERR_BAD_ARG;
+    }
+    for (int i = 0; i < nprob; ++i) {
+        if (nmodes[2 * i] < 1 || nmodes[2 * i + 1] < 1 || ntransf[i] < 1 || npoints[i] < 0) {
+            set_error("ragged problem %d: modes >= 1, ntransf >= 1, npoints >= 0 required", i);
+            return JFA_ERR_BAD_ARG;
+        }
+    }
+    if (!(eps > 0)) {
+        set_error("eps must be positive");
+        return JFA_ERR_BAD_ARG;
+    }
+    jfa_opts o;
+    if (opts) o = *opts;
+    else jfa_default_opts(&o);
+    if (!(o.upsampfac >= 1.0) || (o.modeord != 0 && o.modeord != 1) ||
+        (o.kernel_width != 0 && (o.kernel_width < 2 || o.kernel_width > kMaxNs))) {
+        set_error("bad options: upsampfac >= 1, modeord 0/1, kernel_width 0 or in [2, %d]",
+                  kMaxNs);
+        return JFA_ERR_BAD_ARG;
+    }
+    if (int rc = ensure_rocfft()) return rc;
+    DeviceScope ds;
+    if (o.gpu_device_id >= 0)
+        if (int rc = ds.set(o.gpu_device_id)) return rc;
+
+    auto *r = new jfa_ragged_s();
+    r->nprob = nprob;
+    r->is_double = is_double ? 1 : 0;
+    r->iflag = iflag >= 0 ? 1 : -1;
+    r->eps = eps;
+    r->opts = o;
+    r->stream = (hipStream_t)o.gpu_stream;
+    r->use_graph = use_graph != 0;
+    if (hipGetDevice(&r->device) != hipSuccess) r->device = 0;
+    r->kp = choose_kernel(eps, o.upsampfac, o.width_rule, r->is_double, o.kernel_width);
+    for (int i = 0; i < nprob; ++i) {
+        r->n0.push_back(nmodes[2 * i]);
+        r->n1.push_back(nmodes[2 * i + 1]);
+        r->ntr.push_back(ntransf[i]);
+        r->npts.push_back(npoints[i]);
+    }
+    int rc = layout(r);
+    if (!rc) rc = allocate(r);
+    if (rc) {
+        release(r);
+        delete r;
+        return rc;
+    }
+    *plan_out = r;
+    return r->kp.warn ? JFA_WARN_EPS_TOO_SMALL : JFA_OK;
+}
+
+extern "C" int jfa_ragged_run(jfa_ragged r, const void *const *points, const void *const *c,
+                              void *const *f) {
+    if (!r || !points || !c || !f) {
+        set_error("ragged run: NULL plan or pointer table");
+        return JFA_ERR_BAD_ARG;
+    }
+    const int np = r->nprob;
+    for (int i = 0; i < np; ++i) {
+        if (r->npts[i] > 0 && (!points[i] || !points[np + i] || !c[i])) {
+            set_error("ragged problem %d: NULL points or strengths", i);
+            return JFA_ERR_BAD_ARG;
+        }
+        if (!f[i]) {
+            set_error("ragged problem %d: NULL output", i);
+            return JFA_ERR_BAD_ARG;
+        }
+    }
+    DeviceScope ds;
+    if (int rc = ds.set(r->device)) return rc;
+    hipStream_t s = r->stream;
+    if (r->use_graph) {
+        JFA_HIP_CHECK(hipEventRecord(r->ev_in, r->stream));
+        JFA_HIP_CHECK(hipStreamWaitEvent(r->work, r->ev_in, 0));
+        s = r->work;
+    }
+    // the previous run's table copy must have left the pinned staging buffer
+    JFA_HIP_CHECK(hipEventSynchronize(r->ptrs_copied));
+    for (int i = 0; i < np; ++i) {
+        r->h_ptrs[i] = points[i];
+        r->h_ptrs[np + i] = points[np + i];
+        r->h_ptrs[2 * np + i] = c[i];
+        r->h_ptrs[3 * np + i] = f[i];
+    }
+    JFA_HIP_CHECK(hipMemcpyAsync(r->d_ptrs, r->h_ptrs, sizeof(void *) * 4 * np,
+                                 hipMemcpyHostToDevice, s));
+    JFA_HIP_CHECK(hipEventRecord(r->ptrs_copied, s));
+    int rc = r->is_double ? run_t<double>(r, s) : run_t<float>(r, s);
+    if (rc) return rc;
+    if (r->use_graph) {
+        JFA_HIP_CHECK(hipEventRecord(r->ev_out, r->work));
+        JFA_HIP_CHECK(hipStreamWaitEvent(r->stream, r->ev_out, 0));
+    }
+    return JFA_OK;
+}
+
+extern "C" int jfa_ragged_uses_graph(jfa_ragged r) {
+    return r && r->use_graph && r->graph_state == 1 ? 1 : 0;
+}
+
+extern "C" int jfa_ragged_set_stream(jfa_ragged r, jfa_stream_t stream) {
+    if (!r) {
+        set_error("plan is NULL");
+        return JFA_ERR_BAD_ARG;
+    }
+    if ((hipStream_t)stream == r->stream) return JFA_OK;
+    if (!r->use_graph) {
+        // order the new stream after this plan's previous work
+        hipEvent_t e;
+        JFA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        JFA_HIP_CHECK(hipEventRecord(e, r->stream));
+        JFA_HIP_CHECK(hipStreamWaitEvent((hipStream_t)stream, e, 0));
+        (void)hipEventDestroy(e);
+    }
+    r->stream = (hipStream_t)stream;
+    return JFA_OK;
+}
+
+extern "C" int jfa_ragged_destroy(jfa_ragged r) {
+    if (!r) return JFA_OK;
+    DeviceScope ds;
+    (void)ds.set(r->device);
+    hipStream_t s = r->use_graph ? r->work : r->stream;
+    (void)hipStreamSynchronize(s);
+    release(r);
+    delete r;
+    return JFA_OK;
+}

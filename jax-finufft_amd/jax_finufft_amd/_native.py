@@ -63,6 +63,10 @@ PLAN_API = [
     "jfa_default_opts", "jfa_makeplan", "jfa_setpts", "jfa_execute", "jfa_set_stream",
     "jfa_destroy", "jfa_plan_info", "jfa_last_error", "jfa_cache_size", "jfa_cache_clear",
 ]
+RAGGED_API = [
+    "jfa_ragged_makeplan", "jfa_ragged_run", "jfa_ragged_uses_graph", "jfa_ragged_set_stream",
+    "jfa_ragged_destroy",
+]
 
 
 class NufftError(RuntimeError):
@@ -128,6 +132,15 @@ def lib():
             )
             fn.restype = c_int
         L.jfa_interp_grad.argtypes = [c_void_p] * 5
+        P64 = ctypes.POINTER(c_int64)
+        L.jfa_ragged_makeplan.argtypes = [c_int, c_int, P64, P64, P64, c_int, c_double, c_int,
+                                          ctypes.POINTER(JfaOpts), c_int,
+                                          ctypes.POINTER(c_void_p)]
+        L.jfa_ragged_run.argtypes = [c_void_p, ctypes.POINTER(c_void_p),
+                                     ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]
+        L.jfa_ragged_uses_graph.argtypes = [c_void_p]
+        L.jfa_ragged_set_stream.argtypes = [c_void_p, c_void_p]
+        L.jfa_ragged_destroy.argtypes = [c_void_p]
         _lib = L
         return _lib
 

@@ -1,0 +1,169 @@
+"""Ragged batches of 2D type-1 transforms in one launch sequence.
+
+No reference counterpart: with jax_finufft a batch of independent problems of
+different sizes is a Python loop of ``nufft1`` calls, each of which makes,
+uses and destroys its own plan (src/jax_finufft/ops.py:50-80 ->
+lib/kernels.cc.cu:25-92).  Here the whole batch is one plan: one point sort
+over every problem, one spreading launch over every problem's units, one
+merge, one batched FFT per fine-grid size and one deconvolution, optionally
+replayed from a captured HIP graph.
+
+Conventions are those of ``nufft1``: problem i has points ``x[i], y[i]``
+(real, shape ``(M_i,)``), strengths ``c[i]`` of shape ``(M_i,)`` or
+``(T_i, M_i)`` (T_i stacked strength vectors sharing the points), and output
+modes of shape ``output_shapes[i] = (N1_i, N2_i)`` (or ``(T_i, N1_i, N2_i)``),
+with f[k1, k2] = sum_j c_j exp(i * iflag * (k1 x_j + k2 y_j)).  All problems
+share eps, iflag and dtype.  ``Opts`` fields ``modeord``, ``gpu_upsampfac`` and
+``gpu_kerevalmeth`` apply.
+"""
+
+import ctypes
+import threading
+from collections import OrderedDict
+
+import torch
+
+from . import _native, options
+
+__all__ = ["RaggedPlan", "nufft1_ragged"]
+
+_CDT = {torch.complex64: torch.float32, torch.complex128: torch.float64}
+
+
+class RaggedPlan:
+    """One plan for a fixed list of problem shapes (modes, point counts,
+    stacked transforms).  ``run`` may be called any number of times with new
+    arrays of those shapes."""
+
+    def __init__(self, output_shapes, n_points, ntransf=None, iflag=1, eps=1e-6,
+                 dtype=torch.complex64, opts=None, graph=True, device=None):
+        if dtype not in _CDT:
+            raise TypeError("dtype must be torch.complex64 or torch.complex128")
+        self.shapes = [tuple(int(n) for n in s) for s in output_shapes]
+        if any(len(s) != 2 for s in self.shapes):
+            raise ValueError("ragged batches are 2D: every output shape must be (N1, N2)")
+        self.n_points = [int(m) for m in n_points]
+        self.ntransf = [1] * len(self.shapes) if ntransf is None else [int(t) for t in ntransf]
+        if not (len(self.shapes) == len(self.n_points) == len(self.ntransf)) or not self.shapes:
+            raise ValueError("output_shapes, n_points and ntransf must be non-empty and of "
+                             "equal length")
+        self.dtype = dtype
+        self.real_dtype = _CDT[dtype]
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        self.iflag = 1 if int(iflag) >= 0 else -1
+        self.eps = float(eps)
+        self._plan = ctypes.c_void_p(None)
+        o = options.unpack_opts(opts, 1, True) or options.Opts()
+        native = o.to_native_opts()
+        L = _native.lib()
+        jo = _native.JfaOpts()
+        L.jfa_default_opts(ctypes.byref(jo))
+        # as in the lowering (lowering.py:157-174), the subproblem size is left
+        # to the backend's default
+        for k in ("modeord", "upsampfac", "gpu_kerevalmeth"):
+            setattr(jo, k, native[k])
+        jo.gpu_stream = torch.cuda.current_stream(self.device).cuda_stream
+        n = len(self.shapes)
+        # backend order is x fastest: (N2, N1) per problem
+        nm = (ctypes.c_int64 * (2 * n))(*[v for s in self.shapes for v in (s[1], s[0])])
+        nt = (ctypes.c_int64 * n)(*self.ntransf)
+        npt = (ctypes.c_int64 * n)(*self.n_points)
+        with torch.cuda.device(self.device):
+            rc = L.jfa_ragged_makeplan(2, n, nm, nt, npt, self.iflag, self.eps,
+                                       int(dtype == torch.complex128), ctypes.byref(jo),
+                                       int(bool(graph)), ctypes.byref(self._plan))
+        self.warn = _native.check(rc, "ragged makeplan")
+        self._stream = jo.gpu_stream
+
+    @property
+    def uses_graph(self):
+        return bool(_native.lib().jfa_ragged_uses_graph(self._plan))
+
+    def run(self, points_x, points_y, sources, out=None):
+        """Transform every problem; returns the list of outputs."""
+        n = len(self.shapes)
+        if not (len(points_x) == len(points_y) == len(sources) == n):
+            raise ValueError(f"expected {n} problems")
+        xs, ys, cs, fs = [], [], [], []
+        for i in range(n):
+            x, y, c = points_x[i], points_y[i], sources[i]
+            _native._require_device(x, y, c)
+            m, t = self.n_points[i], self.ntransf[i]
+            if x.dtype != self.real_dtype or y.dtype != self.real_dtype or c.dtype != self.dtype:
+                raise TypeError(f"problem {i}: points must be {self.real_dtype} and strengths "
+                                f"{self.dtype}")
+            if x.numel() != m or y.numel() != m or c.numel() != m * t:
+                raise ValueError(f"problem {i}: expected {m} points and {t} x {m} strengths")
+            xs.append(x.contiguous())
+            ys.append(y.contiguous())
+            cs.append(c.contiguous())
+            shape = (t,) + self.shapes[i] if (c.dim() == 2 or t > 1) else self.shapes[i]
+            if out is not None:
+                f = out[i]
+                if f.shape != shape or f.dtype != self.dtype or not f.is_contiguous():
+                    raise ValueError(f"problem {i}: output must be a contiguous {shape} "
+                                     f"{self.dtype} tensor")
+            else:
+                f = torch.empty(shape, dtype=self.dtype, device=self.device)
+            fs.append(f)
+        P = ctypes.c_void_p
+        # backend x = the user's last axis (y), as the lowering reverses points
+        pts = (P * (2 * n))(*[a.data_ptr() for a in ys], *[a.data_ptr() for a in xs])
+        cp = (P * n)(*[a.data_ptr() for a in cs])
+        fp = (P * n)(*[a.data_ptr() for a in fs])
+        L = _native.lib()
+        with torch.cuda.device(self.device):
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            if s != self._stream:
+                _native.check(L.jfa_ragged_set_stream(self._plan, P(s)), "ragged set_stream")
+                self._stream = s
+            _native.check(L.jfa_ragged_run(self._plan, pts, cp, fp), "ragged run")
+        self._keep = (xs, ys, cs)  # alive until the queued work has read them
+        return fs
+
+    def destroy(self):
+        if self._plan and self._plan.value:
+            _native.lib().jfa_ragged_destroy(self._plan)
+            self._plan = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+_cache = OrderedDict()
+_cache_lock = threading.Lock()
+_CACHE_MAX = 8
+
+
+def nufft1_ragged(output_shapes, sources, points_x, points_y, iflag=1, eps=1e-6, opts=None,
+                  graph=True):
+    """Type-1 transforms of a ragged batch; returns a list of outputs.
+
+    A plan is kept per distinct list of shapes (up to eight), so repeated calls
+    with the same shapes reuse the sort buffers, FFT plans and captured graph.
+    """
+    if not sources:
+        return []
+    dtype = sources[0].dtype
+    n_points = [int(x.numel()) for x in points_x]
+    ntransf = [int(c.numel()) // max(m, 1) if m else (c.shape[0] if c.dim() == 2 else 1)
+               for c, m in zip(sources, n_points)]
+    dev = sources[0].device
+    key = (tuple(tuple(s) for s in output_shapes), tuple(n_points), tuple(ntransf), int(iflag),
+           float(eps), dtype, bool(graph), str(dev), repr(opts))
+    with _cache_lock:
+        plan = _cache.get(key)
+        if plan is None:
+            plan = RaggedPlan(output_shapes, n_points, ntransf, iflag=iflag, eps=eps, dtype=dtype,
+                              opts=opts, graph=graph, device=dev)
+            _cache[key] = plan
+            while len(_cache) > _CACHE_MAX:
+                _cache.popitem(last=False)
+        else:
+            _cache.move_to_end(key)
+    return plan.run(points_x, points_y, sources)

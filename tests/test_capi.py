@@ -34,7 +34,7 @@ def L():
 
 def test_header_declares_plan_api_and_18_entry_points():
     names = declared_symbols()
-    for n in _native.ENTRY_POINTS + _native.PLAN_API:
+    for n in _native.ENTRY_POINTS + _native.PLAN_API + _native.RAGGED_API:
         assert n in names
     assert len([n for n in names if re.fullmatch(r"jfa_nufft[123]d[123]f?", n)]) == 18
 
@@ -116,3 +116,25 @@ def test_bad_arguments_are_errors_not_crashes(L):
     rc, *_ = _params(L, -1.0)
     assert rc == 2  # JFA_ERR_BAD_ARG
     assert b"eps" in L.jfa_last_error()
+
+
+def test_ragged_makeplan_rejects_bad_batches_before_device_work(L):
+    # argument checks run before any device call, so they hold without a GPU
+    P64 = ctypes.c_int64
+    plan = ctypes.c_void_p()
+
+    def make(dim, nprob, modes, ntr, npts, eps=1e-6):
+        nm = (P64 * max(len(modes), 1))(*modes)
+        nt = (P64 * max(len(ntr), 1))(*ntr)
+        npt = (P64 * max(len(npts), 1))(*npts)
+        return L.jfa_ragged_makeplan(dim, nprob, nm, nt, npt, 1, eps, 0, None, 0,
+                                     ctypes.byref(plan))
+
+    assert make(3, 1, [64, 64, 64], [1], [10]) == 6  # JFA_ERR_NOT_SUPPORTED: 2D only
+    assert b"2D" in L.jfa_last_error()
+    assert make(2, 0, [], [], []) == 2
+    assert make(2, 2, [64, 64, 64, 64], [1, 0], [10, 10]) == 2  # ntransf >= 1
+    assert make(2, 1, [64, 64], [1], [-5]) == 2                 # npoints >= 0
+    assert make(2, 1, [64, 0], [1], [5]) == 2                   # modes >= 1
+    assert make(2, 1, [64, 64], [1], [5], eps=0.0) == 2
+    assert plan.value is None
