@@ -95,6 +95,13 @@ class _Primitive(torch.autograd.Function):
         ctx.eps = eps
         ctx.opts = opts
         ctx.save_for_backward(source, *points)
+        ctx.save_for_forward(source, *points)
+
+    @staticmethod
+    def jvp(ctx, *tangents):
+        source, *points = ctx.saved_tensors
+        return _tangent(ctx.nufft_type, tangents[5], tangents[6:], source, points,
+                        ctx.output_shape, ctx.iflag, ctx.eps, ctx.opts)
 
     @staticmethod
     def backward(ctx, grad):
@@ -178,6 +185,69 @@ def _stack(arrs):
 
 def _unstack(a, k):
     return torch.chunk(a, k, dim=1)
+
+
+def _tangent(nufft_type, dsource, dpoints, source, points, output_shape, iflag, eps, opts):
+    """Forward-mode rule of the reference (jvp, ops.py:158-277) on the canonical
+    layout: the tangent terms that share a transform type are stacked into one
+    call, each then scaled (by 1, by i*iflag*k_d over the modes, by the point
+    tangent, or by i*iflag*s_d over the targets) and summed.
+
+      type 1: df = A dc + sum_d (i iflag k_d) A(dx_d c)
+      type 2: dc = B df + sum_d dx_d B(i iflag k_d f)
+      type 3: df = C dc + sum_d (i iflag s_d) C(dx_d c) + sum_d ds_d C(i iflag x_d c)
+    """
+    modeord = _modeord(options.unpack_opts(opts, nufft_type, True))
+    ndim = len(points) // 2 if nufft_type == 3 else len(points)
+
+    def run(src, pts=points):
+        # through the primitive, so that tangents wrapped by torch.func transforms
+        # are unwrapped at their level (and higher-order derivatives compose)
+        return _bind(nufft_type, src.contiguous(), pts, output_shape, iflag, eps, opts)
+
+    out = []
+    scales, args = [], []
+    if dsource is not None:
+        if nufft_type == 2:
+            out.append(run(dsource))
+        else:
+            scales.append(None)
+            args.append(dsource)
+    for d in range(ndim):
+        dx = dpoints[d] if d < len(dpoints) else None
+        if dx is None:
+            continue
+        dxb = dx[:, None, :].to(source.dtype)
+        if nufft_type == 1:
+            scales.append(_freq_weights(tuple(output_shape), d, modeord, iflag, source))
+            args.append(dxb * source)
+        elif nufft_type == 2:
+            n_modes = tuple(source.shape[2:])
+            scales.append(dxb)
+            args.append(_freq_weights(n_modes, d, modeord, iflag, source) * source)
+        else:
+            scales.append((1j * iflag) * points[ndim + d][:, None, :].to(source.dtype))
+            args.append(dxb * source)
+    if nufft_type == 3:
+        sc, ar = [], []
+        for d in range(ndim):
+            ds = dpoints[ndim + d] if ndim + d < len(dpoints) else None
+            if ds is None:
+                continue
+            sc.append(ds[:, None, :].to(source.dtype))
+            ar.append((1j * iflag) * points[d][:, None, :].to(source.dtype) * source)
+        if sc:
+            parts = _unstack(run(_stack(ar)), len(ar))
+            out += [s * p for s, p in zip(sc, parts)]
+    if args:
+        parts = _unstack(run(_stack(args)), len(args))
+        out += [p if s is None else s * p for s, p in zip(scales, parts)]
+    if not out:
+        return None
+    total = out[0]
+    for t in out[1:]:
+        total = total + t
+    return total
 
 
 def _adjoint(nufft_type, grad, source, points, output_shape, iflag, eps, opts, need):
