@@ -39,7 +39,7 @@ MODES = (1024, 1024)
 EPS = 1e-6
 # plan stages timed with HIP events on the plan's stream (units / spread / merge
 # are the strip spreader's piece table, spreading kernel and tile merge)
-STAGES = ("setpts", "zero", "units", "spread", "merge", "fft", "deconv")
+STAGES = ("setpts", "zero", "pack", "units", "spread", "merge", "fft", "deconv")
 
 
 def make_points(M, dist, seed, device):

@@ -111,6 +111,9 @@ int jfa_execute(jfa_plan plan, void *c, void *fk);
  *   val[t][j]   = sum_k fk[t][k] e^{i iflag k.x_j}            (if val != NULL)
  *   grad[d][j]  = sum_t Re(conj(w[t][j]) d val[t][j] / d x_dj)  (if grad != NULL)
  * with d in the plan's x-fastest order and grad laid out d-major (dim x M).
+ * fk == NULL reuses the fine grid of the plan's last type-2 execute (valid when
+ * that execute ran all transforms in one batch), skipping amplify + FFT: the
+ * reverse pass of a type-2 evaluation then costs one interpolation pass.
  * Needs the LDS method (JFA_ERR_NOT_SUPPORTED otherwise). */
 int jfa_interp_grad(jfa_plan plan, const void *fk, const void *w, void *val, void *grad);
 

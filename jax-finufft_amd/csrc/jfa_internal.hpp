@@ -115,6 +115,7 @@ struct RaggedVirt {
     int64_t dk_off;    // 1/psi_hat: dk0 at dk + dk_off, dk1 at dk + dk_off + n0
     int64_t moff;      // the problem's first point in the concatenated order
                        // (sorted indices are concatenated-order indices)
+    int64_t pack_off;  // its first record in the concatenated per-transform records
 };
 // Device tables and sizes of one ragged launch sequence.
 struct RaggedLaunch {
@@ -124,10 +125,11 @@ struct RaggedLaunch {
     const int *vunit_off;      // nv + 1
     const int64_t *cell_blk;   // nv + 1: first merge block (256 cells) per virtual problem
     const int64_t *mode_blk;   // nv + 1: first deconv block (256 modes)
+    const int64_t *pack_start; // nv + 1: first record per virtual problem (prefix of m)
     const void *const *ptrs;   // x[nprob], y[nprob], c[nprob], f[nprob]
     int *bin_start, *piece_start, *npieces;
     int nprob, nv, ns, pmax, keyshift;
-    int64_t mtot, nbins, nunits, cell_blocks, mode_blocks;
+    int64_t mtot, nbins, nunits, cell_blocks, mode_blocks, npack;
 };
 
 // Points after setpts, in bin-sorted order: original index, first tap
@@ -140,19 +142,22 @@ struct SortedPts {
     const T *t[3];
 };
 
-// Points as the 2D strip spreader reads them after its setpts: the sorted keys
-// and original indices, and every point's Horner variables (tx, ty) in input
-// order, so that setpts needs no gather.  key = (bin << kStripKeyShift) | X with
-// bin = bin_off + s*nf1 + l0y (s the strip) and X = l0x - s*SW in [0, SW).
+// Points as the 2D strip spreader reads them: the sorted keys and original
+// indices from its setpts, and one 4-value record per input point built at
+// execute time (launch_strip_pack) from the strength and the Horner variables
+// setpts wrote in input order: pack[4j..4j+3] = (Re c_j, Im c_j, tx_j, ty_j).
+// One record read per point instead of two scattered reads; setpts needs no
+// gather.  key = (bin << kStripKeyShift) | X with bin = bin_off + s*nf1 + l0y
+// (s the strip) and X = l0x - s*SW in [0, SW).
 constexpr int kStripKeyShift = 6;
 template <typename T>
 struct StripPts {
     int64_t M;
     const int *idx;       // sorted position -> input point P (over all problems)
     const unsigned *key;  // sorted keys
-    const T *rec;         // (tx, ty) of input point P at rec[2*P]
+    const T *pack;        // records of the problem's transform, by local index j
     int64_t bin_off;      // the problem's first global bin (0 for one plan)
-    int64_t idx_off;      // the problem's first input point: c index = P - idx_off
+    int64_t idx_off;      // the problem's first input point: j = P - idx_off
 };
 
 // Per-bin bookkeeping for the LDS-subproblem method.
@@ -198,9 +203,13 @@ int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const 
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, const T *coef, int kerevalmeth,
-                        double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
+                        double beta, int64_t pack_stride, T *fw, int64_t fw_stride,
                         T *scratch, int64_t scratch_stride, int ntr, int phases,
                         hipStream_t stream);
+// pack[t][j] = (c[t][j], rec[j]) for t < ntr, j < M (records of 4 reals)
+template <typename T>
+int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int ntr, T *pack,
+                      hipStream_t stream);
 // phases of launch_spread_strip (a bit mask): piece table, spreading, merge
 constexpr int kStripUnits = 1, kStripSpread = 2, kStripMerge = 4, kStripAll = 7;
 // Ragged batch: sort every problem's points at once (global bins), then the
@@ -208,6 +217,8 @@ constexpr int kStripUnits = 1, kStripSpread = 2, kStripMerge = 4, kStripAll = 7;
 template <typename T>
 int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T *rec,
                        hipStream_t stream);
+template <typename T>
+int launch_ragged_pack(const RaggedLaunch &rl, const T *rec, T *pack_all, hipStream_t stream);
 template <typename T>
 int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T *coef,
                         int kerevalmeth, double beta, T *fw_all, T *scratch_all, int phases,

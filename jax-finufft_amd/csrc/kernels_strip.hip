@@ -60,6 +60,10 @@ template <typename T>
 struct alignas(2 * sizeof(T)) Pair {
     T x, y;
 };
+template <typename T>
+struct alignas(4 * sizeof(T)) Quad {
+    T x, y, z, w;
+};
 
 // LDS staging of one wave: ns <= 8 uses the weight pairs, wider kernels the
 // column weights and c * y-weights per point.
@@ -103,9 +107,8 @@ template <typename T, int NS>
 __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T> &pts,
                                            const int *__restrict__ bin_start, int u, int sub,
                                            bool heavy, const T *__restrict__ coef, T beta,
-                                           int horner, const T *__restrict__ c,
-                                           T *__restrict__ fw, T *__restrict__ scratch,
-                                           StripLds<T, NS> &lds) {
+                                           int horner, T *__restrict__ fw,
+                                           T *__restrict__ scratch, StripLds<T, NS> &lds) {
     const int lane = threadIdx.x;
     const int rows_tile = sg.H + NS - 1;
     const PieceRange pr = piece_range(sg, bin_start, u, sub, heavy);
@@ -145,11 +148,11 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
         // stage: lane q evaluates point bb + q
         if (lane < nb) {
             const int p = bb + lane;
-            const int64_t P = pts.idx[p];
+            const int64_t j = pts.idx[p] - pts.idx_off;
             const unsigned key = pts.key[p];
-            const int64_t j = P - pts.idx_off;
-            const T cr = c[2 * j], ci = c[2 * j + 1];
-            const Pair<T> tt = reinterpret_cast<const Pair<T> *>(pts.rec)[P];
+            const Quad<T> rq = reinterpret_cast<const Quad<T> *>(pts.pack)[j];
+            const T cr = rq.x, ci = rq.y;
+            const Pair<T> tt{rq.z, rq.w};
             T kx[NS], ky[NS];
             if (horner) {
                 eval_kernel<T, NS, true>(coef, beta, tt.x, kx);
@@ -232,9 +235,8 @@ template <typename T, int NS>
 __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts<T> &pts,
                                              const int *__restrict__ bin_start, int u, int sub,
                                              bool heavy, const T *__restrict__ coef, T beta,
-                                             int horner, const T *__restrict__ c,
-                                             T *__restrict__ fw, T *__restrict__ scratch,
-                                             StripLds<T, NS> &lds) {
+                                             int horner, T *__restrict__ fw,
+                                             T *__restrict__ scratch, StripLds<T, NS> &lds) {
     static_assert(NS <= 8, "modular ownership needs ns <= 8");
     Pair<T> *s_w = lds.w;
     Pair<T> *s_cky = lds.cky;
@@ -280,11 +282,11 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
         int key = -1;  // (row << 3) | block of the lane's staged point
         if (lane < nb) {
             const int p = bb + lane;
-            const int64_t P = pts.idx[p];
+            const int64_t j = pts.idx[p] - pts.idx_off;
             const unsigned pkey = pts.key[p];
-            const int64_t j = P - pts.idx_off;
-            const T cr = c[2 * j], ci = c[2 * j + 1];
-            const Pair<T> tt = reinterpret_cast<const Pair<T> *>(pts.rec)[P];
+            const Quad<T> rq = reinterpret_cast<const Quad<T> *>(pts.pack)[j];
+            const T cr = rq.x, ci = rq.y;
+            const Pair<T> tt{rq.z, rq.w};
             const int X = (int)(pkey & ((1u << kStripKeyShift) - 1));
             // x-weights, then y-weights: one set of ns values live at a time
             {
@@ -354,14 +356,13 @@ template <typename T, int NS>
 __device__ __forceinline__ void strip_piece(const StripGeom &sg, const StripPts<T> &pts,
                                             const int *__restrict__ bin_start, int u, int sub,
                                             bool heavy, const T *__restrict__ coef, T beta,
-                                            int horner, const T *__restrict__ c,
-                                            T *__restrict__ fw, T *__restrict__ scratch,
-                                            StripLds<T, NS> &lds) {
+                                            int horner, T *__restrict__ fw,
+                                            T *__restrict__ scratch, StripLds<T, NS> &lds) {
     if constexpr (NS <= 8)
-        strip8_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, c, fw, scratch,
+        strip8_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, fw, scratch,
                             lds);
     else
-        wide_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, c, fw, scratch,
+        wide_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, fw, scratch,
                           lds);
 }
 
@@ -452,11 +453,11 @@ template <typename T, int NS>
 __global__ void __launch_bounds__(kStripLanes)
     spread_strip_kernel(StripGeom sg, StripPts<T> pts, const int *__restrict__ bin_start,
                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
-                        const T *__restrict__ coef, T beta, int horner, const T *__restrict__ c,
-                        int64_t c_stride, T *__restrict__ fw, int64_t fw_stride,
-                        T *__restrict__ scratch, int64_t scratch_stride) {
+                        const T *__restrict__ coef, T beta, int horner, int64_t pack_stride,
+                        T *__restrict__ fw, int64_t fw_stride, T *__restrict__ scratch,
+                        int64_t scratch_stride) {
     __shared__ StripLds<T, NS> lds;
-    c += 2 * c_stride * blockIdx.y;
+    pts.pack += 4 * pack_stride * blockIdx.y;
     fw += 2 * fw_stride * blockIdx.y;
     scratch += 2 * scratch_stride * blockIdx.y;
     const int total = *npieces;
@@ -464,9 +465,21 @@ __global__ void __launch_bounds__(kStripLanes)
         const int u = segment_of(piece_start, sg.nunits, piece);
         const int sub = piece - piece_start[u];
         const bool heavy = piece_start[u + 1] - piece_start[u] > 1;
-        strip_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, c, fw, scratch,
+        strip_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, fw, scratch,
                            lds);
     }
+}
+
+// pack[t][j] = (c[t][j], rec[j])
+template <typename T>
+__global__ void __launch_bounds__(256)
+    strip_pack_kernel(const T *__restrict__ c, int64_t c_stride, const T *__restrict__ rec,
+                      int64_t M, T *__restrict__ pack) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    const Pair<T> cv = reinterpret_cast<const Pair<T> *>(c + 2 * c_stride * blockIdx.y)[j];
+    const Pair<T> tv = reinterpret_cast<const Pair<T> *>(rec)[j];
+    reinterpret_cast<Quad<T> *>(pack + 4 * M * blockIdx.y)[j] = Quad<T>{cv.x, cv.y, tv.x, tv.y};
 }
 
 template <typename T, int NS>
@@ -515,8 +528,7 @@ __global__ void __launch_bounds__(kStripLanes)
     ragged_strip_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ vunit_off,
                         int nv, StripPts<T> pts, const int *__restrict__ bin_start,
                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
-                        const T *__restrict__ coef, T beta, int horner,
-                        const void *const *__restrict__ ptrs, int nprob, T *__restrict__ fw_all,
+                        const T *__restrict__ coef, T beta, int horner, T *__restrict__ fw_all,
                         T *__restrict__ scratch_all) {
     __shared__ StripLds<T, NS> lds;
     const int total = *npieces;
@@ -527,14 +539,31 @@ __global__ void __launch_bounds__(kStripLanes)
         const RaggedVirt d = virt[v];
         const int sub = piece - piece_start[gu];
         const bool heavy = piece_start[gu + 1] - piece_start[gu] > 1;
-        const T *c = (const T *)ptrs[2 * nprob + d.prob] + 2 * (int64_t)d.t * d.m;
         StripPts<T> vp = pts;
+        vp.pack = pts.pack + 4 * d.pack_off;
         vp.bin_off = d.bin_off;
         vp.idx_off = d.moff;
         strip_piece<T, NS>(d.sg, vp, bin_start + d.bin_off, gu - d.unit_off, sub, heavy, coef,
-                           beta, horner, c, fw_all + 2 * d.fw_off, scratch_all + 2 * d.tile_off,
+                           beta, horner, fw_all + 2 * d.fw_off, scratch_all + 2 * d.tile_off,
                            lds);
     }
+}
+
+// record q of virtual problem v (pack_start[v] <= q): its problem's strength
+// c_i[t][j] and Horner pair rec[moff + j], j = q - pack_start[v]
+template <typename T>
+__global__ void __launch_bounds__(256)
+    ragged_pack_kernel(const RaggedVirt *__restrict__ virt, const int64_t *__restrict__ pack_start,
+                       int nv, const void *const *__restrict__ ptrs, int nprob,
+                       const T *__restrict__ rec, int64_t npack, T *__restrict__ pack) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= npack) return;
+    const int v = segment_of(pack_start, nv, q);
+    const RaggedVirt &d = virt[v];
+    const int64_t j = q - pack_start[v];
+    const T *c = (const T *)ptrs[2 * nprob + d.prob] + 2 * ((int64_t)d.t * d.m + j);
+    const Pair<T> tv = reinterpret_cast<const Pair<T> *>(rec)[d.moff + j];
+    reinterpret_cast<Quad<T> *>(pack)[q] = Quad<T>{c[0], c[1], tv.x, tv.y};
 }
 
 // block b merges 256 cells of the virtual problem v with cell_blk[v] <= b
@@ -556,7 +585,7 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, const T *coef, int kerevalmeth,
-                        double beta, const T *c, int64_t c_stride, T *fw, int64_t fw_stride,
+                        double beta, int64_t pack_stride, T *fw, int64_t fw_stride,
                         T *scratch, int64_t scratch_stride, int ntr, int phases,
                         hipStream_t stream) {
     if (ntr == 0 || pts.M == 0) return JFA_OK;
@@ -571,7 +600,7 @@ int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *
     JFA_DISPATCH_NS(sg.ns, {
         if (phases & kStripSpread)
             spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
-                sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, c, c_stride, fw,
+                sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, pack_stride, fw,
                 fw_stride, scratch, scratch_stride);
         if (phases & kStripMerge)
             strip_merge_kernel<T, NS_>
@@ -600,7 +629,7 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
         if (phases & kStripSpread)
             ragged_strip_kernel<T, NS_><<<grid, kStripLanes, 0, stream>>>(
                 rl.virt, rl.vunit_off, rl.nv, pts, rl.bin_start, rl.piece_start, rl.npieces, coef,
-                b, kerevalmeth, rl.ptrs, rl.nprob, fw_all, scratch_all);
+                b, kerevalmeth, fw_all, scratch_all);
         if (phases & kStripMerge)
             ragged_merge_kernel<T, NS_><<<(unsigned)rl.cell_blocks, 256, 0, stream>>>(
                 rl.virt, rl.cell_blk, rl.nv, rl.piece_start, scratch_all, fw_all);
@@ -609,11 +638,32 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
     return JFA_OK;
 }
 
+template <typename T>
+int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int ntr, T *pack,
+                      hipStream_t stream) {
+    if (M == 0 || ntr == 0) return JFA_OK;
+    strip_pack_kernel<T><<<dim3((unsigned)((M + 255) / 256), ntr), 256, 0, stream>>>(
+        c, c_stride, rec, M, pack);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
+int launch_ragged_pack(const RaggedLaunch &rl, const T *rec, T *pack_all, hipStream_t stream) {
+    if (rl.npack == 0) return JFA_OK;
+    ragged_pack_kernel<T><<<(unsigned)((rl.npack + 255) / 256), 256, 0, stream>>>(
+        rl.virt, rl.pack_start, rl.nv, rl.ptrs, rl.nprob, rec, rl.npack, pack_all);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
 #define JFA_INST(T)                                                                           \
     template int launch_spread_strip<T>(const StripGeom &, const StripPts<T> &, const int *, \
-                                        int *, int *, const T *, int, double, const T *,      \
-                                        int64_t, T *, int64_t, T *, int64_t, int, int,        \
-                                        hipStream_t);                                         \
+                                        int *, int *, const T *, int, double, int64_t, T *,   \
+                                        int64_t, T *, int64_t, int, int, hipStream_t);        \
+    template int launch_strip_pack<T>(const T *, int64_t, const T *, int64_t, int, T *,       \
+                                      hipStream_t);                                           \
+    template int launch_ragged_pack<T>(const RaggedLaunch &, const T *, T *, hipStream_t);    \
     template int launch_ragged_strip<T>(const RaggedLaunch &, const StripPts<T> &, const T *, \
                                         int, double, T *, T *, int, hipStream_t);
 JFA_INST(float)

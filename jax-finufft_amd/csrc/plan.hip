@@ -664,18 +664,35 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
         if (tm) tm->mark("spread");
         return rc;
     }
-    const StripPts<T> pts{p->M, p->d_idx, p->d_key_out, (const T *)p->d_t, 0, 0};
+    // one record (c, tx, ty) per point and transform of a batch: the spreader's
+    // only scattered read
+    const int64_t M = p->M;
+    const size_t pack_bytes = (size_t)std::min(p->batch, ntr) * M * 4 * sizeof(T);
+    if (pack_bytes > p->pack_bytes) {
+        free_ptr(p->d_pack);
+        if (hipMalloc(&p->d_pack, pack_bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p->pack_bytes = 0;
+            set_error("could not allocate %zu bytes of spreading records", pack_bytes);
+            return JFA_ERR_ALLOC;
+        }
+        p->pack_bytes = pack_bytes;
+    }
+    const StripPts<T> pts{M, p->d_idx, p->d_key_out, (const T *)p->d_pack, 0, 0};
     for (int t0 = 0; t0 < ntr; t0 += p->batch) {
         const int nb = std::min(p->batch, ntr - t0);
+        if (int rc = launch_strip_pack<T>(c + 2 * t0 * c_stride, c_stride, (const T *)p->d_t, M,
+                                          nb, (T *)p->d_pack, p->stream))
+            return rc;
+        if (tm) tm->mark("pack");
         const int phases[3] = {kStripUnits, kStripSpread, kStripMerge};
         const char *names[3] = {"units", "spread", "merge"};
         for (int ph = 0; ph < 3; ++ph) {
             if (!tm && ph > 0) break;
             int rc = launch_spread_strip<T>(
                 p->sg, pts, p->d_bin_start, p->d_piece_start, p->d_npieces, (const T *)p->d_coef,
-                kem, p->kp.beta, c + 2 * t0 * c_stride, c_stride, fw + 2 * t0 * fw_stride,
-                fw_stride, (T *)p->d_strip_scratch, p->strip_scratch_stride, nb,
-                tm ? phases[ph] : kStripAll, p->stream);
+                kem, p->kp.beta, M, fw + 2 * t0 * fw_stride, fw_stride, (T *)p->d_strip_scratch,
+                p->strip_scratch_stride, nb, tm ? phases[ph] : kStripAll, p->stream);
             if (rc) return rc;
             if (tm) tm->mark(names[ph]);
         }
@@ -746,6 +763,8 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
             tm.mark("interp");
         }
     }
+    // the fine grid of a single-batch type-2 pass can serve a later gradient pass
+    p->fw_modes = p->type == 2 && p->ntransf <= p->batch;
     tm.report("execute");
     return rc;
 }
@@ -766,7 +785,9 @@ extern "C" int jfa_execute(jfa_plan p, void *c, void *fk) {
         return JFA_ERR_BAD_ARG;
     }
     if (p->M == 0) {
-        // no points: type 1 output is zero, type 2 has nothing to write
+        // no points: type 1 output is zero, type 2 has nothing to write (and its
+        // fine grid is not computed, so it no longer matches the last modes)
+        p->fw_modes = false;
         if (p->type == 1)
             return launch_fill_zero(fk, (int64_t)(p->is_double ? 16 : 8) * p->Ntot * p->ntransf,
                                     p->stream);
@@ -791,13 +812,16 @@ static int interp_grad_t(jfa_plan p, const T *fk, const T *w, T *val, T *grad) {
     StageTimer tm(p);
     for (int t0 = 0; t0 < p->ntransf; t0 += p->batch) {
         const int nb = std::min(p->batch, p->ntransf - t0);
-        int rc = launch_amplify<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2], p->opts.modeord,
+        int rc = JFA_OK;
+        if (fk) {
+            rc = launch_amplify<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2], p->opts.modeord,
                                    fk + 2 * (int64_t)t0 * p->Ntot, p->Ntot, fw, p->F, nb,
                                    p->stream);
-        if (rc) return rc;
-        tm.mark("amplify");
-        if ((rc = fft_exec(p, fw, nb))) return rc;
-        tm.mark("fft");
+            if (rc) return rc;
+            tm.mark("amplify");
+            if ((rc = fft_exec(p, fw, nb))) return rc;
+            tm.mark("fft");
+        }
         rc = launch_interp_grad<T>(g, pts, bi, (const T *)p->d_coef, fw, p->F, nb,
                                    w ? w + 2 * (int64_t)t0 * M : nullptr, M,
                                    val ? val + 2 * (int64_t)t0 * M : nullptr, M, grad, M, t0 > 0,
@@ -805,6 +829,7 @@ static int interp_grad_t(jfa_plan p, const T *fk, const T *w, T *val, T *grad) {
         if (rc) return rc;
         tm.mark("interp_grad");
     }
+    p->fw_modes = p->ntransf <= p->batch;
     tm.report("interp_grad");
     return JFA_OK;
 }
@@ -823,8 +848,13 @@ extern "C" int jfa_interp_grad(jfa_plan p, const void *fk, const void *w, void *
         set_error("interp_grad needs the LDS-tile method (gpu_method 0 or 2)");
         return JFA_ERR_NOT_SUPPORTED;
     }
-    if ((grad && !w) || (p->M > 0 && !fk)) {
+    if (grad && !w) {
         set_error("interp_grad: weights are required for the gradient");
+        return JFA_ERR_BAD_ARG;
+    }
+    if (!fk && !p->fw_modes) {
+        set_error("interp_grad without modes needs the fine grid of a preceding single-batch "
+                  "type-2 execute on this plan");
         return JFA_ERR_BAD_ARG;
     }
     DeviceGuard dg;
@@ -873,7 +903,8 @@ extern "C" int jfa_destroy(jfa_plan p) {
     for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_key_in,
                      (void **)&p->d_val_in, (void **)&p->d_key_out, &p->d_sort_temp, &p->d_t,
                      (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub,
-                     (void **)&p->d_piece_start, (void **)&p->d_npieces, &p->d_strip_scratch})
+                     (void **)&p->d_piece_start, (void **)&p->d_npieces, &p->d_strip_scratch,
+                     &p->d_pack})
         free_ptr(*q);
     jfa::type3_free(p);
     for (hipEvent_t e : p->timing_events) (void)hipEventDestroy(e);

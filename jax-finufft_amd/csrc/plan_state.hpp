@@ -36,6 +36,8 @@ struct jfa_plan_s {
     void *d_coef = nullptr;            // Horner coefficients (nc x ns)
     void *d_dk[3] = {nullptr, nullptr, nullptr};  // 1 / psi_hat per dim
     void *d_fw = nullptr;              // batch x F complex fine grid
+    bool fw_modes = false;             // d_fw holds the amplified, transformed modes of
+                                       // every transform of the last type-2 pass
 
     // rocFFT
     std::map<int, rocfft_plan> fft_plans;  // by batch count
@@ -58,6 +60,8 @@ struct jfa_plan_s {
     void *d_strip_scratch = nullptr;
     size_t strip_scratch_bytes = 0;
     int64_t strip_scratch_stride = 0;  // complex cells per transform
+    void *d_pack = nullptr;            // batch x M records (c, tx, ty)
+    size_t pack_bytes = 0;
 
     // type 3
     jfa_plan inner = nullptr;     // type-2 plan from the spread grid to the targets

@@ -34,13 +34,13 @@ struct jfa_ragged_s {
     std::vector<int64_t> npts, ntr, n0, n1;
     std::vector<RaggedProb> probs;
     std::vector<RaggedVirt> virt;
-    std::vector<int64_t> moff, cell_blk, mode_blk;
+    std::vector<int64_t> moff, cell_blk, mode_blk, pack_start;
     std::vector<int> vunit_off;
     RaggedLaunch rl{};
 
     // device tables
     void *d_probs = nullptr, *d_virt = nullptr, *d_moff = nullptr, *d_vunit_off = nullptr,
-         *d_cell_blk = nullptr, *d_mode_blk = nullptr;
+         *d_cell_blk = nullptr, *d_mode_blk = nullptr, *d_pack_start = nullptr;
     void *d_ptrs = nullptr;
     const void **h_ptrs = nullptr;  // pinned staging of the pointer table
     hipEvent_t ptrs_copied = nullptr;
@@ -55,9 +55,9 @@ struct jfa_ragged_s {
     size_t sort_temp_bytes = 0;
     int *d_bin_start = nullptr, *d_piece_start = nullptr, *d_npieces = nullptr;
 
-    // fine grids and strip scratch (complex cells)
+    // fine grids and strip scratch (complex cells); spreading records (c, tx, ty)
     int64_t fw_cells = 0, tile_cells = 0;
-    void *d_fw = nullptr, *d_scratch = nullptr;
+    void *d_fw = nullptr, *d_scratch = nullptr, *d_pack = nullptr;
 
     // one batched rocFFT per fine-grid size
     struct FftClass {
@@ -106,7 +106,7 @@ void release(jfa_ragged r) {
                      (void **)&r->d_l0, &r->d_t, (void **)&r->d_key_in, (void **)&r->d_val_in,
                      (void **)&r->d_key_out, &r->d_sort_temp, (void **)&r->d_bin_start,
                      (void **)&r->d_piece_start, (void **)&r->d_npieces, &r->d_fw, &r->d_scratch,
-                     &r->d_fft_work})
+                     &r->d_fft_work, &r->d_pack_start, &r->d_pack})
         free_dev(*q);
     if (r->h_ptrs) (void)hipHostFree((void *)r->h_ptrs);
     r->h_ptrs = nullptr;
@@ -200,6 +200,7 @@ int layout(jfa_ragged r) {
     r->vunit_off.assign(1, 0);
     r->cell_blk.assign(1, 0);
     r->mode_blk.assign(1, 0);
+    r->pack_start.assign(1, 0);
     r->fft.clear();
     int64_t fw = 0, tiles = 0;
     for (int i : order) {
@@ -221,7 +222,9 @@ int layout(jfa_ragged r) {
             v.n1 = r->n1[i];
             v.dk_off = dk_off[i];
             v.moff = r->moff[i];
+            v.pack_off = r->pack_start.back();
             r->virt.push_back(v);
+            r->pack_start.push_back(r->pack_start.back() + r->npts[i]);
             fw += F;
             tiles += (int64_t)sg[i].nunits * (sg[i].H + ns - 1) * 64;
             r->vunit_off.push_back(r->vunit_off.back() + sg[i].nunits);
@@ -248,6 +251,7 @@ int layout(jfa_ragged r) {
     rl.nunits = r->vunit_off.back();
     rl.cell_blocks = r->cell_blk.back();
     rl.mode_blocks = r->mode_blk.back();
+    rl.npack = r->pack_start.back();
     return JFA_OK;
 }
 
@@ -260,6 +264,7 @@ int allocate(jfa_ragged r) {
     if (int rc = upload_vec(r->vunit_off, &r->d_vunit_off)) return rc;
     if (int rc = upload_vec(r->cell_blk, &r->d_cell_blk)) return rc;
     if (int rc = upload_vec(r->mode_blk, &r->d_mode_blk)) return rc;
+    if (int rc = upload_vec(r->pack_start, &r->d_pack_start)) return rc;
     JFA_HIP_CHECK(hipMalloc(&r->d_ptrs, sizeof(void *) * 4 * np));
     JFA_HIP_CHECK(hipHostMalloc((void **)&r->h_ptrs, sizeof(void *) * 4 * np, 0));
     JFA_HIP_CHECK(hipEventCreateWithFlags(&r->ptrs_copied, hipEventDisableTiming));
@@ -301,7 +306,8 @@ int allocate(jfa_ragged r) {
     JFA_HIP_CHECK(hipMalloc(&r->d_piece_start, sizeof(int) * (r->rl.nunits + 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_npieces, sizeof(int) * 4));
     if (hipMalloc(&r->d_fw, 2 * real * std::max<int64_t>(r->fw_cells, 1)) != hipSuccess ||
-        hipMalloc(&r->d_scratch, 2 * real * std::max<int64_t>(r->tile_cells, 1)) != hipSuccess) {
+        hipMalloc(&r->d_scratch, 2 * real * std::max<int64_t>(r->tile_cells, 1)) != hipSuccess ||
+        hipMalloc(&r->d_pack, 4 * real * std::max<int64_t>(r->rl.npack, 1)) != hipSuccess) {
         (void)hipGetLastError();
         set_error("could not allocate the ragged batch's fine grids (%lld cells) and tiles",
                   (long long)r->fw_cells);
@@ -347,6 +353,7 @@ int allocate(jfa_ragged r) {
     rl.vunit_off = (const int *)r->d_vunit_off;
     rl.cell_blk = (const int64_t *)r->d_cell_blk;
     rl.mode_blk = (const int64_t *)r->d_mode_blk;
+    rl.pack_start = (const int64_t *)r->d_pack_start;
     rl.ptrs = (const void *const *)r->d_ptrs;
     rl.bin_start = r->d_bin_start;
     rl.piece_start = r->d_piece_start;
@@ -354,10 +361,11 @@ int allocate(jfa_ragged r) {
     return JFA_OK;
 }
 
-// sorted keys and concatenated-order indices; Horner variables by index
+// sorted keys and concatenated-order indices; the spreading records of every
+// virtual problem (each kernel offsets them by the problem's pack_off)
 template <typename T>
 StripPts<T> sorted(jfa_ragged r) {
-    return StripPts<T>{r->mtot, r->d_idx, r->d_key_out, (const T *)r->d_t, 0, 0};
+    return StripPts<T>{r->mtot, r->d_idx, r->d_key_out, (const T *)r->d_pack, 0, 0};
 }
 
 // Everything after the sort: zero the fine grids, spread, merge, FFT, deconvolve.
@@ -365,6 +373,7 @@ template <typename T>
 int enqueue_execute(jfa_ragged r, hipStream_t s) {
     const int kem = r->opts.gpu_kerevalmeth ? 1 : 0;
     if (int rc = launch_fill_zero(r->d_fw, (int64_t)(2 * sizeof(T)) * r->fw_cells, s)) return rc;
+    if (int rc = launch_ragged_pack<T>(r->rl, (const T *)r->d_t, (T *)r->d_pack, s)) return rc;
     if (int rc = launch_ragged_strip<T>(r->rl, sorted<T>(r), (const T *)r->d_coef, kem,
                                         r->kp.beta, (T *)r->d_fw, (T *)r->d_scratch, kStripAll,
                                         s))

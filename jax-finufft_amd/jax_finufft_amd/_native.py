@@ -350,6 +350,24 @@ class Plan:
             check(lib().jfa_execute(self._plan, c_void_p(c.data_ptr()),
                                     c_void_p(fk.data_ptr())), "execute")
 
+    def interp_grad(self, w, fk=None, want_value=False):
+        """Fused gradient pass of a type-2 plan on its current points: returns
+        (value or None, [d value-weighted derivative per user dimension]).
+        ``fk=None`` reuses the fine grid of this plan's last execute."""
+        M = self.M
+        val = torch.empty((self.ntransf, M), dtype=self.dtype, device=self.device) \
+            if want_value else None
+        grad = torch.empty((self.ndim, M), dtype=self.real_dtype, device=self.device)
+        w = w.contiguous()
+        ptr = lambda t: c_void_p(t.data_ptr()) if t is not None else c_void_p(None)  # noqa: E731
+        with torch.cuda.device(self.device):
+            self._sync_stream()
+            rc = lib().jfa_interp_grad(self._plan, ptr(fk), ptr(w), ptr(val), ptr(grad))
+        if rc == 6:
+            raise NotImplementedError(last_error())
+        check(rc, "interp_grad")
+        return val, [grad[self.ndim - 1 - d] for d in range(self.ndim)]
+
     def destroy(self):
         if self._plan and self._plan.value:
             lib().jfa_destroy(self._plan)
@@ -360,3 +378,41 @@ class Plan:
             self.destroy()
         except Exception:
             pass
+
+
+class PlanPool:
+    """Idle guru plans by configuration, so that paths that hold a plan across
+    calls (the reverse pass of a type-2 evaluation keeps its plan, with its
+    sorted points and fine grid, from forward to backward) do not make a plan
+    (rocFFT plan, tables, scratch) per call."""
+
+    def __init__(self, max_idle=4):
+        self._idle = {}
+        self._lock = threading.Lock()
+        self.max_idle = max_idle
+
+    def take(self, key, make):
+        with self._lock:
+            lst = self._idle.get(key)
+            if lst:
+                return lst.pop()
+        return make()
+
+    def give(self, key, plan):
+        plan._keep = ()  # drop the last call's point arrays
+        with self._lock:
+            lst = self._idle.setdefault(key, [])
+            if len(lst) < self.max_idle:
+                lst.append(plan)
+                return
+        plan.destroy()
+
+    def clear(self):
+        with self._lock:
+            plans = [p for lst in self._idle.values() for p in lst]
+            self._idle.clear()
+        for p in plans:
+            p.destroy()
+
+
+plan_pool = PlanPool()

@@ -116,7 +116,111 @@ class _Primitive(torch.autograd.Function):
         return _batch(in_dims[5:], nufft_type, output_shape, iflag, eps, opts, source, points)
 
 
+class _PlanLease:
+    """A pooled type-2 plan held from a forward evaluation to its backward pass;
+    returned to the pool when the backward pass ends or the graph is freed."""
+
+    def __init__(self, key, plan):
+        self.key, self.plan = key, plan
+
+    def release(self):
+        if self.plan is not None:
+            _native.plan_pool.give(self.key, self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class _Type2Kept(torch.autograd.Function):
+    """Type 2 on one set of points whose reverse pass reuses the forward plan:
+    its sorted points and its fine grid FFT(amplify(f)) give the point
+    gradients in one interpolation pass (jfa_interp_grad with fk = NULL), with
+    no second sort, amplify or FFT.  grad_f is the usual type-1 transform.
+    Used under plain autograd when coordinates need gradients and the fused
+    adjoint is on; torch.func transforms use _Primitive."""
+
+    @staticmethod
+    def forward(ctx, iflag, eps, opts, source, *points):
+        fopts = options.unpack_opts(opts, 2, True) or options.Opts()
+        native = fopts.to_cufinufft_opts()
+        n_modes = tuple(source.shape[2:])
+        T = source.shape[1]
+        fields = dict(modeord=int(native.modeord), upsampfac=float(native.upsampfac),
+                      gpu_method=int(native.gpu_method), gpu_sort=int(native.gpu_sort),
+                      gpu_kerevalmeth=int(native.gpu_kerevalmeth),
+                      gpu_maxbatchsize=int(native.gpu_maxbatchsize), debug=int(native.debug))
+        key = (n_modes, T, int(iflag), float(eps), source.dtype, str(source.device),
+               tuple(sorted(fields.items())))
+        plan = _native.plan_pool.take(key, lambda: _native.Plan(
+            2, n_modes, iflag=iflag, ntransf=T, eps=eps, dtype=source.dtype, device=source.device,
+            **fields))
+        lease = _PlanLease(key, plan)
+        M = points[0].shape[1]
+        plan.setpts(*[p[0] for p in points])
+        out = torch.empty((1, T, M), dtype=source.dtype, device=source.device)
+        plan.execute(out, source.contiguous())
+        ctx.lease = lease
+        ctx.iflag, ctx.eps, ctx.opts = iflag, eps, opts
+        ctx.save_for_backward(source, *points)
+        ctx.save_for_forward(source, *points)
+        return out
+
+    @staticmethod
+    def jvp(ctx, iflag_t, eps_t, opts_t, dsource, *dpoints):
+        source, *points = ctx.saved_tensors
+        return _tangent(2, dsource, dpoints, source, points, None, ctx.iflag, ctx.eps, ctx.opts)
+
+    @staticmethod
+    def backward(ctx, grad):
+        source, *points = ctx.saved_tensors
+        need_src, need_pts = ctx.needs_input_grad[3], ctx.needs_input_grad[4:]
+        ndim = len(points)
+        out = [None] * (1 + ndim)
+        grad = grad.contiguous()
+        if any(need_pts):
+            grads = None
+            if ctx.lease.plan is not None:
+                try:
+                    # sum_t Re(conj(g) dc/dx_d) from the forward's grid and points
+                    _, grads = ctx.lease.plan.interp_grad(grad[0], fk=None)
+                    grads = [g[None] for g in grads]
+                except NotImplementedError:
+                    grads = None  # the plan interpolates without LDS tiles
+            if grads is None:
+                # a second backward pass over a retained graph, or no fused kernel
+                grads = _adjoint(2, grad, source, points, None, ctx.iflag, ctx.eps, ctx.opts,
+                                 (False, *need_pts))[1:]
+            for d in range(ndim):
+                if need_pts[d]:
+                    out[1 + d] = grads[d]
+        ctx.lease.release()
+        if need_src:
+            bopts = options.unpack_opts(ctx.opts, 1, False)
+            out[0] = _bind(1, grad, points, tuple(source.shape[2:]), -ctx.iflag, ctx.eps, bopts)
+        return (None, None, None, *out)
+
+
+def _keep_plan(nufft_type, source, points, opts):
+    if nufft_type != 2 or source.shape[0] != 1 or not torch.is_grad_enabled():
+        return False
+    if not any(p.requires_grad for p in points):
+        return False
+    if torch._C._are_functorch_transforms_active():
+        return False
+    fopts = options.unpack_opts(opts, 2, True)
+    if fopts is None:
+        return True
+    # the gradient pass needs the LDS-tile interpolation (gpu_method 0 or 2)
+    return bool(getattr(fopts, "gpu_fused_adjoint", True)) and int(fopts.gpu_method) in (0, 2)
+
+
 def _bind(nufft_type, source, points, output_shape, iflag, eps, opts):
+    if _keep_plan(nufft_type, source, points, opts):
+        return _Type2Kept.apply(iflag, eps, opts, source, *points)
     return _Primitive.apply(nufft_type, output_shape, iflag, eps, opts, source, *points)
 
 
