@@ -115,7 +115,6 @@ struct RaggedVirt {
     int64_t dk_off;    // 1/psi_hat: dk0 at dk + dk_off, dk1 at dk + dk_off + n0
     int64_t moff;      // the problem's first point in the concatenated order
                        // (sorted indices are concatenated-order indices)
-    int64_t pack_off;  // its first record in the concatenated per-transform records
 };
 // Device tables and sizes of one ragged launch sequence.
 struct RaggedLaunch {
@@ -125,11 +124,11 @@ struct RaggedLaunch {
     const int *vunit_off;      // nv + 1
     const int64_t *cell_blk;   // nv + 1: first merge block (256 cells) per virtual problem
     const int64_t *mode_blk;   // nv + 1: first deconv block (256 modes)
-    const int64_t *pack_start; // nv + 1: first record per virtual problem (prefix of m)
     const void *const *ptrs;   // x[nprob], y[nprob], c[nprob], f[nprob]
     int *bin_start, *piece_start, *npieces;
+    int *blocksum;             // 2 * ceil(nunits / 256) + 1 ints of piece-scan scratch
     int nprob, nv, ns, pmax, keyshift;
-    int64_t mtot, nbins, nunits, cell_blocks, mode_blocks, npack;
+    int64_t mtot, nbins, nunits, cell_blocks, mode_blocks;
 };
 
 // Points after setpts, in bin-sorted order: original index, first tap
@@ -155,7 +154,10 @@ struct StripPts {
     int64_t M;
     const int *idx;       // sorted position -> input point P (over all problems)
     const unsigned *key;  // sorted keys
-    const T *pack;        // records of the problem's transform, by local index j
+    const T *pack;        // records of the problem's transform, by local index j;
+                          // NULL: read c[j] (this transform's strengths) and rec[P]
+    const T *c;
+    const T *rec;         // (tx, ty) of input point P at rec[2*P], written by setpts
     int64_t bin_off;      // the problem's first global bin (0 for one plan)
     int64_t idx_off;      // the problem's first input point: j = P - idx_off
 };
@@ -217,8 +219,6 @@ constexpr int kStripUnits = 1, kStripSpread = 2, kStripMerge = 4, kStripAll = 7;
 template <typename T>
 int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T *rec,
                        hipStream_t stream);
-template <typename T>
-int launch_ragged_pack(const RaggedLaunch &rl, const T *rec, T *pack_all, hipStream_t stream);
 template <typename T>
 int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T *coef,
                         int kerevalmeth, double beta, T *fw_all, T *scratch_all, int phases,

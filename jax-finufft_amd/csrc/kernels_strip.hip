@@ -82,6 +82,17 @@ struct StripLds<T, NS, false> {
     int ly[kStripLanes];
 };
 
+// Strength and Horner pair of input point P: from the packed record, or from
+// the strengths (by index within the problem) and the setpts pairs (by P).
+template <typename T>
+__device__ __forceinline__ Quad<T> point_record(const StripPts<T> &pts, int64_t P) {
+    const int64_t j = P - pts.idx_off;
+    if (pts.pack) return reinterpret_cast<const Quad<T> *>(pts.pack)[j];
+    const Pair<T> cv = reinterpret_cast<const Pair<T> *>(pts.c)[j];
+    const Pair<T> tv = reinterpret_cast<const Pair<T> *>(pts.rec)[P];
+    return Quad<T>{cv.x, cv.y, tv.x, tv.y};
+}
+
 // The point range [pb, pe) of piece `sub` of unit u, and its strip / rows.
 struct PieceRange {
     int ox, r0, pb, pe;
@@ -148,9 +159,8 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
         // stage: lane q evaluates point bb + q
         if (lane < nb) {
             const int p = bb + lane;
-            const int64_t j = pts.idx[p] - pts.idx_off;
             const unsigned key = pts.key[p];
-            const Quad<T> rq = reinterpret_cast<const Quad<T> *>(pts.pack)[j];
+            const Quad<T> rq = point_record(pts, pts.idx[p]);
             const T cr = rq.x, ci = rq.y;
             const Pair<T> tt{rq.z, rq.w};
             T kx[NS], ky[NS];
@@ -282,9 +292,8 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
         int key = -1;  // (row << 3) | block of the lane's staged point
         if (lane < nb) {
             const int p = bb + lane;
-            const int64_t j = pts.idx[p] - pts.idx_off;
             const unsigned pkey = pts.key[p];
-            const Quad<T> rq = reinterpret_cast<const Quad<T> *>(pts.pack)[j];
+            const Quad<T> rq = point_record(pts, pts.idx[p]);
             const T cr = rq.x, ci = rq.y;
             const Pair<T> tt{rq.z, rq.w};
             const int X = (int)(pkey & ((1u << kStripKeyShift) - 1));
@@ -508,19 +517,67 @@ __global__ void __launch_bounds__(1024)
 // slices of the concatenated buffers; its strengths come from the pointer
 // table (c of problem i at ptrs[2*nprob + i], transform-major).
 
-__global__ void __launch_bounds__(1024)
-    ragged_units_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ vunit_off,
-                        int nv, const int *__restrict__ bin_start, int *piece_start,
-                        int *npieces) {
+// Piece table of every virtual problem's units in three launches: counts and
+// per-block sums, a scan of the block sums, then the block-local scans.
+__device__ __forceinline__ int ragged_unit_pieces(const RaggedVirt *__restrict__ virt,
+                                                  const int *__restrict__ vunit_off, int nv,
+                                                  const int *__restrict__ bin_start, int gu) {
+    const int v = segment_of(vunit_off, nv, gu);
+    const RaggedVirt &d = virt[v];
+    return unit_pieces(d.sg, bin_start + d.bin_off, gu - d.unit_off);
+}
+
+// exclusive scan of one value per thread over a 256-thread block; returns the
+// block total in *total
+__device__ __forceinline__ int block_scan256(int v, int *total) {
+    __shared__ int sa[256];
+    const int tid = threadIdx.x;
+    sa[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int w = tid >= off ? sa[tid - off] : 0;
+        __syncthreads();
+        sa[tid] += w;
+        __syncthreads();
+    }
+    *total = sa[255];
+    return sa[tid] - v;
+}
+
+__global__ void __launch_bounds__(256)
+    ragged_units_count_kernel(const RaggedVirt *__restrict__ virt,
+                              const int *__restrict__ vunit_off, int nv,
+                              const int *__restrict__ bin_start, int *piece_start,
+                              int *blocksum) {
     const int nu = vunit_off[nv];
+    const int gu = blockIdx.x * 256 + threadIdx.x;
+    const int c = gu < nu ? ragged_unit_pieces(virt, vunit_off, nv, bin_start, gu) : 0;
+    if (gu < nu) piece_start[gu] = c;
+    int total;
+    block_scan256(c, &total);
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
+}
+
+// one workgroup: blockoff = exclusive scan of the nblk block sums (nblk + 1
+// entries); total also at piece_start[nu] and *npieces
+__global__ void __launch_bounds__(1024)
+    ragged_units_blocks_kernel(const int *__restrict__ vunit_off, int nv, int nblk,
+                               const int *__restrict__ blocksum, int *blockoff,
+                               int *piece_start, int *npieces) {
     piece_scan(
-        nu,
-        [&](int gu) {
-            const int v = segment_of(vunit_off, nv, gu);
-            const RaggedVirt &d = virt[v];
-            return unit_pieces(d.sg, bin_start + d.bin_off, gu - d.unit_off);
-        },
-        piece_start, npieces);
+        nblk, [&](int b) { return blocksum[b]; }, blockoff, npieces);
+    if (threadIdx.x == 1023) piece_start[vunit_off[nv]] = blockoff[nblk];
+}
+
+__global__ void __launch_bounds__(256)
+    ragged_units_scan_kernel(const int *__restrict__ vunit_off, int nv,
+                             const int *__restrict__ blockoff, int *piece_start) {
+    const int nu = vunit_off[nv];
+    const int gu = blockIdx.x * 256 + threadIdx.x;
+    const int c = gu < nu ? piece_start[gu] : 0;
+    int total;
+    const int e = block_scan256(c, &total);
+    if (gu < nu) piece_start[gu] = blockoff[blockIdx.x] + e;
 }
 
 template <typename T, int NS>
@@ -528,7 +585,8 @@ __global__ void __launch_bounds__(kStripLanes)
     ragged_strip_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ vunit_off,
                         int nv, StripPts<T> pts, const int *__restrict__ bin_start,
                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
-                        const T *__restrict__ coef, T beta, int horner, T *__restrict__ fw_all,
+                        const T *__restrict__ coef, T beta, int horner,
+                        const void *const *__restrict__ ptrs, int nprob, T *__restrict__ fw_all,
                         T *__restrict__ scratch_all) {
     __shared__ StripLds<T, NS> lds;
     const int total = *npieces;
@@ -540,30 +598,13 @@ __global__ void __launch_bounds__(kStripLanes)
         const int sub = piece - piece_start[gu];
         const bool heavy = piece_start[gu + 1] - piece_start[gu] > 1;
         StripPts<T> vp = pts;
-        vp.pack = pts.pack + 4 * d.pack_off;
+        vp.c = (const T *)ptrs[2 * nprob + d.prob] + 2 * (int64_t)d.t * d.m;
         vp.bin_off = d.bin_off;
         vp.idx_off = d.moff;
         strip_piece<T, NS>(d.sg, vp, bin_start + d.bin_off, gu - d.unit_off, sub, heavy, coef,
                            beta, horner, fw_all + 2 * d.fw_off, scratch_all + 2 * d.tile_off,
                            lds);
     }
-}
-
-// record q of virtual problem v (pack_start[v] <= q): its problem's strength
-// c_i[t][j] and Horner pair rec[moff + j], j = q - pack_start[v]
-template <typename T>
-__global__ void __launch_bounds__(256)
-    ragged_pack_kernel(const RaggedVirt *__restrict__ virt, const int64_t *__restrict__ pack_start,
-                       int nv, const void *const *__restrict__ ptrs, int nprob,
-                       const T *__restrict__ rec, int64_t npack, T *__restrict__ pack) {
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= npack) return;
-    const int v = segment_of(pack_start, nv, q);
-    const RaggedVirt &d = virt[v];
-    const int64_t j = q - pack_start[v];
-    const T *c = (const T *)ptrs[2 * nprob + d.prob] + 2 * ((int64_t)d.t * d.m + j);
-    const Pair<T> tv = reinterpret_cast<const Pair<T> *>(rec)[d.moff + j];
-    reinterpret_cast<Quad<T> *>(pack)[q] = Quad<T>{c[0], c[1], tv.x, tv.y};
 }
 
 // block b merges 256 cells of the virtual problem v with cell_blk[v] <= b
@@ -618,8 +659,16 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
     // no points anywhere: the zeroed fine grids are already the result
     if (rl.nv == 0 || pts.M == 0) return JFA_OK;
     if (phases & kStripUnits) {
-        ragged_units_kernel<<<1, 1024, 0, stream>>>(rl.virt, rl.vunit_off, rl.nv, rl.bin_start,
-                                                    rl.piece_start, rl.npieces);
+        const unsigned nblk = (unsigned)((rl.nunits + 255) / 256);
+        ragged_units_count_kernel<<<nblk, 256, 0, stream>>>(rl.virt, rl.vunit_off, rl.nv,
+                                                            rl.bin_start, rl.piece_start,
+                                                            rl.blocksum);
+        int *blockoff = rl.blocksum + nblk;  // nblk + 1 entries
+        ragged_units_blocks_kernel<<<1, 1024, 0, stream>>>(rl.vunit_off, rl.nv, (int)nblk,
+                                                           rl.blocksum, blockoff,
+                                                           rl.piece_start, rl.npieces);
+        ragged_units_scan_kernel<<<nblk, 256, 0, stream>>>(rl.vunit_off, rl.nv, blockoff,
+                                                           rl.piece_start);
         JFA_HIP_CHECK(hipGetLastError());
     }
     const int64_t ub = rl.nunits + pts.M / rl.pmax + rl.nv;
@@ -629,7 +678,7 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
         if (phases & kStripSpread)
             ragged_strip_kernel<T, NS_><<<grid, kStripLanes, 0, stream>>>(
                 rl.virt, rl.vunit_off, rl.nv, pts, rl.bin_start, rl.piece_start, rl.npieces, coef,
-                b, kerevalmeth, fw_all, scratch_all);
+                b, kerevalmeth, rl.ptrs, rl.nprob, fw_all, scratch_all);
         if (phases & kStripMerge)
             ragged_merge_kernel<T, NS_><<<(unsigned)rl.cell_blocks, 256, 0, stream>>>(
                 rl.virt, rl.cell_blk, rl.nv, rl.piece_start, scratch_all, fw_all);
@@ -648,22 +697,12 @@ int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int
     return JFA_OK;
 }
 
-template <typename T>
-int launch_ragged_pack(const RaggedLaunch &rl, const T *rec, T *pack_all, hipStream_t stream) {
-    if (rl.npack == 0) return JFA_OK;
-    ragged_pack_kernel<T><<<(unsigned)((rl.npack + 255) / 256), 256, 0, stream>>>(
-        rl.virt, rl.pack_start, rl.nv, rl.ptrs, rl.nprob, rec, rl.npack, pack_all);
-    JFA_HIP_CHECK(hipGetLastError());
-    return JFA_OK;
-}
-
 #define JFA_INST(T)                                                                           \
     template int launch_spread_strip<T>(const StripGeom &, const StripPts<T> &, const int *, \
                                         int *, int *, const T *, int, double, int64_t, T *,   \
                                         int64_t, T *, int64_t, int, int, hipStream_t);        \
     template int launch_strip_pack<T>(const T *, int64_t, const T *, int64_t, int, T *,       \
                                       hipStream_t);                                           \
-    template int launch_ragged_pack<T>(const RaggedLaunch &, const T *, T *, hipStream_t);    \
     template int launch_ragged_strip<T>(const RaggedLaunch &, const StripPts<T> &, const T *, \
                                         int, double, T *, T *, int, hipStream_t);
 JFA_INST(float)

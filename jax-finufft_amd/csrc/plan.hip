@@ -678,7 +678,7 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
         }
         p->pack_bytes = pack_bytes;
     }
-    const StripPts<T> pts{M, p->d_idx, p->d_key_out, (const T *)p->d_pack, 0, 0};
+    const StripPts<T> pts{M, p->d_idx, p->d_key_out, (const T *)p->d_pack, nullptr, nullptr, 0, 0};
     for (int t0 = 0; t0 < ntr; t0 += p->batch) {
         const int nb = std::min(p->batch, ntr - t0);
         if (int rc = launch_strip_pack<T>(c + 2 * t0 * c_stride, c_stride, (const T *)p->d_t, M,

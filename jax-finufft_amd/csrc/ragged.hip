@@ -34,13 +34,13 @@ struct jfa_ragged_s {
     std::vector<int64_t> npts, ntr, n0, n1;
     std::vector<RaggedProb> probs;
     std::vector<RaggedVirt> virt;
-    std::vector<int64_t> moff, cell_blk, mode_blk, pack_start;
+    std::vector<int64_t> moff, cell_blk, mode_blk;
     std::vector<int> vunit_off;
     RaggedLaunch rl{};
 
     // device tables
     void *d_probs = nullptr, *d_virt = nullptr, *d_moff = nullptr, *d_vunit_off = nullptr,
-         *d_cell_blk = nullptr, *d_mode_blk = nullptr, *d_pack_start = nullptr;
+         *d_cell_blk = nullptr, *d_mode_blk = nullptr;
     void *d_ptrs = nullptr;
     const void **h_ptrs = nullptr;  // pinned staging of the pointer table
     hipEvent_t ptrs_copied = nullptr;
@@ -53,11 +53,12 @@ struct jfa_ragged_s {
     unsigned *d_key_in = nullptr, *d_val_in = nullptr, *d_key_out = nullptr;
     void *d_sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
-    int *d_bin_start = nullptr, *d_piece_start = nullptr, *d_npieces = nullptr;
+    int *d_bin_start = nullptr, *d_piece_start = nullptr, *d_npieces = nullptr,
+        *d_blocksum = nullptr;
 
-    // fine grids and strip scratch (complex cells); spreading records (c, tx, ty)
+    // fine grids and strip scratch (complex cells)
     int64_t fw_cells = 0, tile_cells = 0;
-    void *d_fw = nullptr, *d_scratch = nullptr, *d_pack = nullptr;
+    void *d_fw = nullptr, *d_scratch = nullptr;
 
     // one batched rocFFT per fine-grid size
     struct FftClass {
@@ -106,7 +107,7 @@ void release(jfa_ragged r) {
                      (void **)&r->d_l0, &r->d_t, (void **)&r->d_key_in, (void **)&r->d_val_in,
                      (void **)&r->d_key_out, &r->d_sort_temp, (void **)&r->d_bin_start,
                      (void **)&r->d_piece_start, (void **)&r->d_npieces, &r->d_fw, &r->d_scratch,
-                     &r->d_fft_work, &r->d_pack_start, &r->d_pack})
+                     &r->d_fft_work, (void **)&r->d_blocksum})
         free_dev(*q);
     if (r->h_ptrs) (void)hipHostFree((void *)r->h_ptrs);
     r->h_ptrs = nullptr;
@@ -200,7 +201,6 @@ int layout(jfa_ragged r) {
     r->vunit_off.assign(1, 0);
     r->cell_blk.assign(1, 0);
     r->mode_blk.assign(1, 0);
-    r->pack_start.assign(1, 0);
     r->fft.clear();
     int64_t fw = 0, tiles = 0;
     for (int i : order) {
@@ -222,9 +222,7 @@ int layout(jfa_ragged r) {
             v.n1 = r->n1[i];
             v.dk_off = dk_off[i];
             v.moff = r->moff[i];
-            v.pack_off = r->pack_start.back();
             r->virt.push_back(v);
-            r->pack_start.push_back(r->pack_start.back() + r->npts[i]);
             fw += F;
             tiles += (int64_t)sg[i].nunits * (sg[i].H + ns - 1) * 64;
             r->vunit_off.push_back(r->vunit_off.back() + sg[i].nunits);
@@ -251,7 +249,6 @@ int layout(jfa_ragged r) {
     rl.nunits = r->vunit_off.back();
     rl.cell_blocks = r->cell_blk.back();
     rl.mode_blocks = r->mode_blk.back();
-    rl.npack = r->pack_start.back();
     return JFA_OK;
 }
 
@@ -264,7 +261,6 @@ int allocate(jfa_ragged r) {
     if (int rc = upload_vec(r->vunit_off, &r->d_vunit_off)) return rc;
     if (int rc = upload_vec(r->cell_blk, &r->d_cell_blk)) return rc;
     if (int rc = upload_vec(r->mode_blk, &r->d_mode_blk)) return rc;
-    if (int rc = upload_vec(r->pack_start, &r->d_pack_start)) return rc;
     JFA_HIP_CHECK(hipMalloc(&r->d_ptrs, sizeof(void *) * 4 * np));
     JFA_HIP_CHECK(hipHostMalloc((void **)&r->h_ptrs, sizeof(void *) * 4 * np, 0));
     JFA_HIP_CHECK(hipEventCreateWithFlags(&r->ptrs_copied, hipEventDisableTiming));
@@ -305,9 +301,9 @@ int allocate(jfa_ragged r) {
     JFA_HIP_CHECK(hipMalloc(&r->d_bin_start, sizeof(int) * (r->rl.nbins + 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_piece_start, sizeof(int) * (r->rl.nunits + 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_npieces, sizeof(int) * 4));
+    JFA_HIP_CHECK(hipMalloc(&r->d_blocksum, sizeof(int) * (2 * ((r->rl.nunits + 255) / 256) + 1)));
     if (hipMalloc(&r->d_fw, 2 * real * std::max<int64_t>(r->fw_cells, 1)) != hipSuccess ||
-        hipMalloc(&r->d_scratch, 2 * real * std::max<int64_t>(r->tile_cells, 1)) != hipSuccess ||
-        hipMalloc(&r->d_pack, 4 * real * std::max<int64_t>(r->rl.npack, 1)) != hipSuccess) {
+        hipMalloc(&r->d_scratch, 2 * real * std::max<int64_t>(r->tile_cells, 1)) != hipSuccess) {
         (void)hipGetLastError();
         set_error("could not allocate the ragged batch's fine grids (%lld cells) and tiles",
                   (long long)r->fw_cells);
@@ -353,19 +349,23 @@ int allocate(jfa_ragged r) {
     rl.vunit_off = (const int *)r->d_vunit_off;
     rl.cell_blk = (const int64_t *)r->d_cell_blk;
     rl.mode_blk = (const int64_t *)r->d_mode_blk;
-    rl.pack_start = (const int64_t *)r->d_pack_start;
     rl.ptrs = (const void *const *)r->d_ptrs;
     rl.bin_start = r->d_bin_start;
     rl.piece_start = r->d_piece_start;
     rl.npieces = r->d_npieces;
+    rl.blocksum = r->d_blocksum;
     return JFA_OK;
 }
 
-// sorted keys and concatenated-order indices; the spreading records of every
-// virtual problem (each kernel offsets them by the problem's pack_off)
+// sorted keys and concatenated-order indices; the spreader reads each virtual
+// problem's strengths through the pointer table and the Horner pairs setpts
+// wrote.  (Packing (c, tx, ty) records first, as one plan does, costs more than
+// it saves here: most problems are small enough for their strengths to stay in
+// cache, and a problem with T transforms would pack its pairs T times.)
 template <typename T>
 StripPts<T> sorted(jfa_ragged r) {
-    return StripPts<T>{r->mtot, r->d_idx, r->d_key_out, (const T *)r->d_pack, 0, 0};
+    return StripPts<T>{r->mtot, r->d_idx, r->d_key_out, nullptr, nullptr, (const T *)r->d_t,
+                       0, 0};
 }
 
 // Everything after the sort: zero the fine grids, spread, merge, FFT, deconvolve.
@@ -373,7 +373,6 @@ template <typename T>
 int enqueue_execute(jfa_ragged r, hipStream_t s) {
     const int kem = r->opts.gpu_kerevalmeth ? 1 : 0;
     if (int rc = launch_fill_zero(r->d_fw, (int64_t)(2 * sizeof(T)) * r->fw_cells, s)) return rc;
-    if (int rc = launch_ragged_pack<T>(r->rl, (const T *)r->d_t, (T *)r->d_pack, s)) return rc;
     if (int rc = launch_ragged_strip<T>(r->rl, sorted<T>(r), (const T *)r->d_coef, kem,
                                         r->kp.beta, (T *)r->d_fw, (T *)r->d_scratch, kStripAll,
                                         s))
