@@ -251,6 +251,127 @@ def ragged_batch(dev, steps=5, warmup=2):
     }
 
 
+def _timed_plan(plan, pts, src, out, steps, warmup, targets=()):
+    import torch
+
+    for _ in range(warmup):
+        plan.setpts(*pts, targets=targets)
+        plan.execute(*((src, out) if plan.nufft_type != 2 else (out, src)))
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        plan.setpts(*pts, targets=targets)
+        plan.execute(*((src, out) if plan.nufft_type != 2 else (out, src)))
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def other_transforms(dev, steps=5, warmup=2):
+    """The other transforms of the benchmark configurations, each as setpts +
+    execute on a warm plan, with a sampled accuracy check in float64:
+      type2_2d: 2D type 2, complex64, 1024x1024 complex-normal modes, M = 1e7
+                uniform points, eps = 1e-6;
+      type2_3d: 3D type 2, complex64, 256^3 complex-normal modes, a koosh-ball
+                trajectory of 8192 spokes x 2048 samples (M = 16,777,216;
+                spoke directions on the golden-angle sphere, radius linear in
+                [-pi, pi)), eps = 1e-5;
+      type3_3d: 3D type 3, complex64, 2^20 sources and 2^20 targets uniform in
+                [-16, 16]^3, eps = 1e-6, upsampling factor 2.  At range 50 the
+                outer grid is 3200^3 cells (sigma 2) or 2048^3 (sigma 1.25) and
+                the inner type-2 grid twice / 1.25x that, past the plan's
+                2^32-cell limit; sigma 1.25 in single precision also cannot
+                reach 1e-6 in 3D type 3 (its deconvolution divides by psi_hat
+                down to ~exp(-0.4 beta) per axis).  So the range is reduced to
+                16 (outer grid 336^3, inner 672^3)."""
+    import numpy as np
+    import torch
+
+    from jax_finufft_amd import Plan
+
+    res = {}
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+
+    def cn(shape):
+        return torch.complex(torch.randn(shape, generator=g, device=dev),
+                             torch.randn(shape, generator=g, device=dev))
+
+    # 2D type 2
+    M = M_POINTS
+    pts = [(torch.rand(M, generator=g, device=dev) * 2 - 1) * math.pi for _ in range(2)]
+    f = cn(MODES)
+    c = torch.empty(M, dtype=torch.complex64, device=dev)
+    plan = Plan(2, MODES, iflag=-1, eps=EPS, dtype=torch.complex64, device=dev)
+    ms = _timed_plan(plan, pts, f, c, steps, warmup)
+    sel = torch.randint(0, M, (64,), generator=g, device=dev)
+    k1 = torch.arange(-MODES[0] // 2, MODES[0] // 2, device=dev, dtype=torch.float64)
+    k2 = torch.arange(-MODES[1] // 2, MODES[1] // 2, device=dev, dtype=torch.float64)
+    ex = []
+    for j in sel.tolist():
+        ph = -(k1[:, None] * pts[0][j].double() + k2[None, :] * pts[1][j].double())
+        ex.append(torch.sum(f.to(torch.complex128) * torch.polar(torch.ones_like(ph), ph)))
+    ex = torch.stack(ex)
+    err = float(torch.linalg.norm(c[sel].to(torch.complex128) - ex) / torch.linalg.norm(ex))
+    res["type2_2d"] = {"ms": round(ms, 4), "nu_pts_per_s": round(M / (ms * 1e-3), 1),
+                       "rel_l2_sampled_points": err, "M": M, "modes": list(MODES), "eps": EPS}
+    del plan, pts, f, c
+
+    # 3D type 2 on a koosh ball
+    spokes, samples, n3 = 8192, 2048, 256
+    s = torch.arange(spokes, device=dev, dtype=torch.float64)
+    zc = 1 - (2 * s + 1) / spokes
+    rc = torch.sqrt(1 - zc * zc)
+    phi = s * math.pi * (3 - math.sqrt(5))
+    dirs = torch.stack([rc * torch.cos(phi), rc * torch.sin(phi), zc], 1)
+    r = -math.pi + 2 * math.pi * torch.arange(samples, device=dev, dtype=torch.float64) / samples
+    pts = [(dirs[:, d:d + 1] * r[None, :]).reshape(-1).to(torch.float32).contiguous()
+           for d in range(3)]
+    M3 = pts[0].numel()
+    f3 = cn((n3, n3, n3))
+    c3 = torch.empty(M3, dtype=torch.complex64, device=dev)
+    plan = Plan(2, (n3, n3, n3), iflag=-1, eps=1e-5, dtype=torch.complex64, device=dev)
+    ms = _timed_plan(plan, pts, f3, c3, max(2, steps // 2), 1)
+    sel = torch.randint(0, M3, (8,), generator=g, device=dev)
+    k = torch.arange(-n3 // 2, n3 // 2, device=dev, dtype=torch.float64)
+    ex = []
+    for j in sel.tolist():
+        ph = -(k[:, None, None] * pts[0][j].double() + k[None, :, None] * pts[1][j].double()
+               + k[None, None, :] * pts[2][j].double())
+        ex.append(torch.sum(f3.to(torch.complex128) * torch.polar(torch.ones_like(ph), ph)))
+    ex = torch.stack(ex)
+    err = float(torch.linalg.norm(c3[sel].to(torch.complex128) - ex) / torch.linalg.norm(ex))
+    res["type2_3d"] = {"ms": round(ms, 4), "nu_pts_per_s": round(M3 / (ms * 1e-3), 1),
+                       "rel_l2_sampled_points": err, "M": M3, "modes": [n3] * 3, "eps": 1e-5,
+                       "points": "koosh ball, 8192 golden-angle spokes x 2048 samples"}
+    del plan, pts, f3, c3
+
+    # 3D type 3
+    n = 1 << 20
+    src = [(torch.rand(n, generator=g, device=dev) * 2 - 1) * 16 for _ in range(3)]
+    tgt = [(torch.rand(n, generator=g, device=dev) * 2 - 1) * 16 for _ in range(3)]
+    cc = cn(n)
+    out3 = torch.empty(n, dtype=torch.complex64, device=dev)
+    plan = Plan(3, (), iflag=-1, eps=1e-6, dtype=torch.complex64, device=dev, ndim=3)
+    ms = _timed_plan(plan, src, cc, out3, max(2, steps // 2), 1, targets=tgt)
+    sel = torch.randint(0, n, (16,), generator=g, device=dev)
+    ex = []
+    for k_ in sel.tolist():
+        ph = -(src[0].double() * tgt[0][k_].double() + src[1].double() * tgt[1][k_].double()
+               + src[2].double() * tgt[2][k_].double())
+        ex.append(torch.sum(cc.to(torch.complex128) * torch.polar(torch.ones_like(ph), ph)))
+    ex = torch.stack(ex)
+    err = float(torch.linalg.norm(out3[sel].to(torch.complex128) - ex) / torch.linalg.norm(ex))
+    res["type3_3d"] = {"ms": round(ms, 4), "nu_pts_per_s": round(n / (ms * 1e-3), 1),
+                       "rel_l2_sampled_targets": err, "sources": n, "targets": n, "eps": 1e-6,
+                       "range": "[-16, 16]^3", "upsampfac": 2.0,
+                       "fine_grid": list(plan.info()["nf"])}
+    del plan
+    return res
+
+
 def traffic_from_profiles():
     """HBM bytes per spread launch from committed rocprofv3 --pmc summaries."""
     path = os.path.join(ROOT, "profiles", "spread_2d_t1_f32_traffic.json")
@@ -301,6 +422,7 @@ def main():
     ap.add_argument("--no-clustered", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-ragged", action="store_true")
+    ap.add_argument("--no-others", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, copy 512 MiB device-to-device 3 times: the "
                          "known-size reference that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -395,6 +517,8 @@ def main():
         out["grad"] = grad_ratio(dev)
     if not args.no_ragged:
         out["ragged"] = ragged_batch(dev)
+    if not args.no_others:
+        out["other_transforms"] = other_transforms(dev)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, info["ns"])

@@ -180,7 +180,9 @@ struct SortScratch {
 // Launchers (kernels_*.hip).  All queue work on `stream` and return a JFA code.
 size_t sort_temp_bytes(int64_t M);
 // Bin, radix-sort and gather M points: idx/l0/tv in bin order, bin_start
-// (nbins + 1) and, when sub_start is non-null, the LDS-subproblem prefix.
+// (nbins + 1) and, when sub_start is non-null, the LDS-subproblem prefix
+// (sub_start holds nbins + 1 + 2 * ceil(nbins / 256) + 1 ints: the prefix, then
+// scan scratch).
 template <typename T>
 int launch_sort_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
                        const SortScratch &s, int *bin_start, int *sub_start, int *nsub,

@@ -102,39 +102,80 @@ __global__ void __launch_bounds__(256)
     bin_start[b] = (int)lo;
 }
 
-// One workgroup: exclusive prefix of the subproblems per bin
-// (ceil(count / maxsub)); total at sub_start[nbins] and *nsub.
+// Exclusive prefix of the subproblems per bin (ceil(count / maxsub)) in three
+// launches: per-bin counts with 256-bin block sums, a one-workgroup scan of
+// the block sums, then the block-local scans.  Total at sub_start[nbins] and
+// *nsub.  blk holds 2 * ceil(nbins / 256) + 1 ints.
+__device__ __forceinline__ int block_excl_scan256(int v, int *total) {
+    __shared__ int sa[256];
+    const int tid = threadIdx.x;
+    sa[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int w = tid >= off ? sa[tid - off] : 0;
+        __syncthreads();
+        sa[tid] += w;
+        __syncthreads();
+    }
+    *total = sa[255];
+    const int e = sa[tid] - v;
+    __syncthreads();
+    return e;
+}
+
+__global__ void __launch_bounds__(256)
+    sub_count_kernel(int64_t nbins, const int *__restrict__ bin_start, int maxsub, int *sub_start,
+                     int *blk) {
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int c = 0;
+    if (b < nbins) {
+        const int cnt = bin_start[b + 1] - bin_start[b];
+        c = (cnt + maxsub - 1) / maxsub;
+        sub_start[b] = c;
+    }
+    int total;
+    block_excl_scan256(c, &total);
+    if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+// one workgroup: blk[nblk ..] = exclusive scan of blk[0, nblk)
 __global__ void __launch_bounds__(1024)
-    sub_scan_kernel(int64_t nbins, const int *__restrict__ bin_start, int *sub_start, int *nsub,
-                    int maxsub) {
+    sub_blocks_kernel(int64_t nbins, int nblk, int *blk, int *sub_start, int *nsub) {
     __shared__ int sa[1024];
     const int tid = threadIdx.x;
-    const int64_t per = (nbins + 1023) / 1024;
-    const int64_t lo = tid * per;
-    const int64_t hi = lo + per < nbins ? lo + per : nbins;
-    int s = 0;
-    for (int64_t i = lo; i < hi; ++i) {
-        const int cnt = bin_start[i + 1] - bin_start[i];
-        s += (cnt + maxsub - 1) / maxsub;
-    }
-    sa[tid] = s;
+    const int per = (nblk + 1023) / 1024;
+    const int lo = min(tid * per, nblk), hi = min(lo + per, nblk);
+    int a = 0;
+    for (int i = lo; i < hi; ++i) a += blk[i];
+    sa[tid] = a;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        int v = tid >= off ? sa[tid - off] : 0;
+        const int v = tid >= off ? sa[tid - off] : 0;
         __syncthreads();
         sa[tid] += v;
         __syncthreads();
     }
-    int e = sa[tid] - s;
-    for (int64_t i = lo; i < hi; ++i) {
-        sub_start[i] = e;
-        const int cnt = bin_start[i + 1] - bin_start[i];
-        e += (cnt + maxsub - 1) / maxsub;
+    int e = sa[tid] - a;
+    int *off = blk + nblk;
+    for (int i = lo; i < hi; ++i) {
+        const int c = blk[i];
+        off[i] = e;
+        e += c;
     }
     if (tid == 1023) {
+        off[nblk] = sa[1023];
         sub_start[nbins] = sa[1023];
         *nsub = sa[1023];
     }
+}
+
+__global__ void __launch_bounds__(256)
+    sub_apply_kernel(int64_t nbins, int nblk, const int *__restrict__ blk, int *sub_start) {
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int c = b < nbins ? sub_start[b] : 0;
+    int total;
+    const int e = block_excl_scan256(c, &total);
+    if (b < nbins) sub_start[b] = blk[nblk + blockIdx.x] + e;
 }
 
 // Sorted position p holds point j = idx[p] (written by the sort): store its
@@ -271,7 +312,12 @@ int launch_sort_points(const GridGeom &g, int64_t M, const T *x, const T *y, con
         JFA_HIP_CHECK(hipGetLastError());
     }
     if (sub_start) {
-        sub_scan_kernel<<<1, 1024, 0, stream>>>(g.nbins, bin_start, sub_start, nsub, maxsub);
+        // block sums live after the nbins + 1 prefix entries
+        const int nblk = (int)((g.nbins + 255) / 256);
+        int *blk = sub_start + g.nbins + 1;
+        sub_count_kernel<<<nblk, 256, 0, stream>>>(g.nbins, bin_start, maxsub, sub_start, blk);
+        sub_blocks_kernel<<<1, 1024, 0, stream>>>(g.nbins, nblk, blk, sub_start, nsub);
+        sub_apply_kernel<<<nblk, 256, 0, stream>>>(g.nbins, nblk, blk, sub_start);
         JFA_HIP_CHECK(hipGetLastError());
     }
     if (M > 0) {

@@ -494,6 +494,7 @@ extern "C" int jfa_makeplan(int type, int dim, const int64_t *nmodes, int iflag,
             jfa_destroy(p);
             return JFA_ERR_BAD_ARG;
         }
+        p->method_by_density = o.gpu_method == JFA_METHOD_AUTO && p->method == JFA_METHOD_SM;
         if (p->method == JFA_METHOD_STRIP) {
             if (int rc = setup_strip(p)) {
                 jfa_destroy(p);
@@ -544,7 +545,8 @@ int jfa::reserve_points(jfa_plan p, int64_t M) {
             free_ptr(reinterpret_cast<void *&>(*q));
         const size_t nb = (size_t)p->geom.nbins + 1;
         JFA_HIP_CHECK(hipMalloc(&p->d_bin_start, sizeof(int) * nb));
-        JFA_HIP_CHECK(hipMalloc(&p->d_sub_start, sizeof(int) * nb));
+        // the subproblem prefix, then the block sums of its scan
+        JFA_HIP_CHECK(hipMalloc(&p->d_sub_start, sizeof(int) * (nb + 2 * ((nb + 255) / 256) + 2)));
         if (!p->d_nsub) JFA_HIP_CHECK(hipMalloc(&p->d_nsub, sizeof(int) * 4));
         p->cap_bins = (int64_t)nb;
     }
@@ -570,8 +572,26 @@ int jfa::reserve_points(jfa_plan p, int64_t M) {
     return JFA_OK;
 }
 
+// Auto method between LDS subproblem tiles and global atomics in sorted order,
+// from a cost model evaluated once the point count is known: the tile method
+// moves every bin's (bin + halo) tile between HBM and LDS once per
+// subproblem, the global method touches w^d fine-grid cells per point (in
+// sorted order, mostly through L2).  Tiles win when the taps outnumber the tile
+// cells, i.e. when the points are dense enough to amortize the halo.
+static void choose_method_by_density(jfa_plan p, int64_t M) {
+    if (!p->method_by_density) return;
+    const GridGeom &g = p->geom;
+    double taps = (double)M, tiles = (double)g.nbins;
+    for (int d = 0; d < p->dim; ++d) {
+        taps *= g.ns;
+        tiles *= g.bs[d] + g.ns - 1;
+    }
+    p->method = taps >= tiles ? JFA_METHOD_SM : JFA_METHOD_GM_SORT;
+}
+
 template <typename T>
 static int sort_points(jfa_plan p, int64_t M, const T *x, const T *y, const T *z) {
+    choose_method_by_density(p, M);
     const GridGeom &g = p->geom;
     if (p->method == JFA_METHOD_STRIP) {
         SortScratch s{p->d_key_in, p->d_val_in, p->d_key_out, p->d_sort_temp,

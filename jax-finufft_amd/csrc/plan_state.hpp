@@ -17,6 +17,8 @@ struct jfa_plan_s {
     jfa::KernelParams kp;
     jfa::GridGeom geom{};
     int method = JFA_METHOD_SM;
+    bool method_by_density = false;  // auto method: LDS tiles or global atomics, chosen
+                                     // at setpts from the point density
     int maxsub = 4096;
     hipStream_t stream = nullptr;
     int device = 0;

@@ -173,6 +173,7 @@ static int type3_setpts_t(jfa_plan p, int64_t M, const T *const xs[3], int64_t N
         g.ymajor = 0;
         p->method = p->opts.gpu_method == JFA_METHOD_GM_SORT || !fits ? JFA_METHOD_GM_SORT
                                                                       : JFA_METHOD_SM;
+        p->method_by_density = p->opts.gpu_method == JFA_METHOD_AUTO && fits;
         const int om = p->opts.gpu_method;
         if ((om == JFA_METHOD_AUTO || om == JFA_METHOD_STRIP) && strip_applicable(p)) {
             int rc = setup_strip(p);

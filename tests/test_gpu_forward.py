@@ -249,3 +249,28 @@ def test_bad_inputs_raise():
         nufft1((4,), c.to(torch.complex128), x)
     with pytest.raises(ValueError):
         nufft3(c, x, x, x)
+
+
+@pytest.mark.parametrize("ndim,prec", product([1, 2, 3], ["f32", "f64"]))
+def test_upsampfac_125_all_types(ndim, prec):
+    # the sigma = 1.25 width / shape rule and grids, through every type.  The
+    # tolerances are the ones sigma = 1.25 can reach: deconvolution divides the
+    # band-edge modes by psi_hat ~ exp(-0.4 beta) (vs exp(-0.13 beta) at sigma =
+    # 2), which amplifies fp32 rounding of the fine grid ~1000x, and eps = 1e-9
+    # in fp64 would need a kernel wider than the 16-tap cap
+    rng = np.random.default_rng(12 + ndim)
+    M = 2000
+    eps = 1e-4 if prec == "f32" else 1e-8
+    x = uniform_points(rng, ndim, M, prec)
+    c = cnormal(rng, M, prec)
+    n_modes = MODES[ndim]
+    f = cnormal(rng, n_modes, prec)
+    opts = Opts(gpu_upsampfac=1.25)
+    got1 = host(nufft1(n_modes, dev(c), *[dev(a) for a in x], eps=eps, opts=opts))
+    assert rel_l2(got1, oracle.type1(x, c, n_modes, 1)) <= eps
+    got2 = host(nufft2(dev(f), *[dev(a) for a in x], eps=eps, opts=opts))
+    assert rel_l2(got2, oracle.type2(f, x, -1)) <= eps
+    s = [rng.uniform(-6.0, 6.0, 900).astype(NPR[prec]) for _ in range(ndim)]
+    xs = [(rng.uniform(-4.0, 4.0, M)).astype(NPR[prec]) for _ in range(ndim)]
+    got3 = host(nufft3(dev(c), *[dev(a) for a in xs], *[dev(a) for a in s], eps=eps, opts=opts))
+    assert rel_l2(got3, oracle.type3(c, xs, s, -1)) <= eps
