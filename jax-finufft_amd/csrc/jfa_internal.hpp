@@ -75,7 +75,9 @@ struct GridGeom {
     int ns;              // kernel width
     int dim;
     int ymajor;          // 2D bins ordered y fastest (bin = by + nb1*bx), for strips
-    int keyshift;        // sort key = bin << keyshift | (l0x mod bs0); 0 = bin only
+    int keyshift;        // sort key = bin << keyshift | (first-tap offset bits); 0 = bin only
+    int subw[3];         // bits of the first tap's offset within its bin per dimension
+                         // (LDS-tile / global methods without a gather; see SortedPts)
 };
 
 // Geometry of the 2D strip spreader (kernels_strip.hip): strips of SW = 64 -
@@ -133,12 +135,17 @@ struct RaggedLaunch {
 
 // Points after setpts, in bin-sorted order: original index, first tap
 // (wrapped into [0, nf)) and the Horner variable t = 2*frac - 1 per dim.
+// Either as arrays in sorted order (l0, t), or, when key != NULL, without a
+// gather: the sorted key holds the bin and the first taps' offsets within it
+// (GridGeom::keyshift / subw) and rec[j*dim + d] holds t_d of input point j.
 template <typename T>
 struct SortedPts {
     int64_t M;
     const int *idx;
     const int *l0[3];
     const T *t[3];
+    const unsigned *key;
+    const T *rec;
 };
 
 // Points as the 2D strip spreader reads them: the sorted keys and original
@@ -187,6 +194,13 @@ template <typename T>
 int launch_sort_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
                        const SortScratch &s, int *bin_start, int *sub_start, int *nsub,
                        int maxsub, int *idx, int *l0, T *tv, hipStream_t stream);
+// The same without the gather (g.keyshift = sum of g.subw > 0): keys with the
+// offset bits, the Horner variables in input order in rec (dim per point), the
+// sorted keys left in s.key_out.
+template <typename T>
+int launch_sort_points_keyed(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                             const SortScratch &s, int *bin_start, int *sub_start, int *nsub,
+                             int maxsub, int *idx, T *rec, hipStream_t stream);
 // Same per-point data in input order (no sort).
 template <typename T>
 int launch_fold_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,

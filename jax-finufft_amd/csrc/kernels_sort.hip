@@ -74,6 +74,31 @@ __global__ void __launch_bounds__(256)
     rec[2 * j + 1] = ty;
 }
 
+// Key with the offset bits, and the Horner variables, of point j.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+    bin_keys_rec_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
+                        const T *__restrict__ z, unsigned *key, unsigned *val, T *rec) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    const T *xs[3] = {x, y, z};
+    int64_t b = 0, stride = 1;
+    unsigned off = 0;
+    int sh = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const Folded f = fold_point((double)xs[d][j], g.nf[d], g.scale[d], g.shift[d], g.ns);
+        const int bd = f.l0w >> g.subw[d];
+        b += bd * stride;
+        stride *= g.nb[d];
+        off |= (unsigned)(f.l0w - (bd << g.subw[d])) << sh;
+        sh += g.subw[d];
+        rec[j * DIM + d] = (T)f.t;
+    }
+    key[j] = (unsigned)((b << g.keyshift) | off);
+    val[j] = (unsigned)j;
+}
+
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256)
     bin_keys_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
@@ -330,6 +355,38 @@ int launch_sort_points(const GridGeom &g, int64_t M, const T *x, const T *y, con
 }
 
 template <typename T>
+int launch_sort_points_keyed(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
+                             const SortScratch &s, int *bin_start, int *sub_start, int *nsub,
+                             int maxsub, int *idx, T *rec, hipStream_t stream) {
+    if (M > 0) {
+        const unsigned blocks = (unsigned)((M + 255) / 256);
+        JFA_DISPATCH_DIM(g.dim, bin_keys_rec_kernel<T, DIM_><<<blocks, 256, 0, stream>>>(
+                                    g, M, x, y, z, s.key_in, s.val_in, rec));
+        JFA_HIP_CHECK(hipGetLastError());
+        // only the bin bits are sorted; the offset bits ride along
+        size_t bytes = s.temp_bytes;
+        JFA_HIP_CHECK(rocprim::radix_sort_pairs(s.temp, bytes, s.key_in, s.key_out, s.val_in,
+                                                (unsigned *)idx, (unsigned)M, g.keyshift,
+                                                g.keyshift + key_bits(g.nbins), stream));
+    }
+    {
+        const unsigned blocks = (unsigned)((g.nbins + 1 + 255) / 256);
+        bin_bounds_kernel<<<blocks, 256, 0, stream>>>(M, g.nbins, g.keyshift, s.key_out,
+                                                      bin_start);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    if (sub_start) {
+        const int nblk = (int)((g.nbins + 255) / 256);
+        int *blk = sub_start + g.nbins + 1;
+        sub_count_kernel<<<nblk, 256, 0, stream>>>(g.nbins, bin_start, maxsub, sub_start, blk);
+        sub_blocks_kernel<<<1, 1024, 0, stream>>>(g.nbins, nblk, blk, sub_start, nsub);
+        sub_apply_kernel<<<nblk, 256, 0, stream>>>(g.nbins, nblk, blk, sub_start);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    return JFA_OK;
+}
+
+template <typename T>
 int launch_fold_points(const GridGeom &g, int64_t M, const T *x, const T *y, const T *z,
                        int *idx, int *l0, T *tv, hipStream_t stream) {
     if (M == 0) return JFA_OK;
@@ -387,6 +444,9 @@ int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T
     template int launch_sort_points_strip<T>(const GridGeom &, int64_t, const T *, const T *, \
                                              const SortScratch &, int *, int *, T *,          \
                                              hipStream_t);                                    \
+    template int launch_sort_points_keyed<T>(const GridGeom &, int64_t, const T *, const T *, \
+                                             const T *, const SortScratch &, int *, int *,     \
+                                             int *, int, int *, T *, hipStream_t);             \
     template int launch_sort_points<T>(const GridGeom &, int64_t, const T *, const T *,      \
                                        const T *, const SortScratch &, int *, int *, int *,  \
                                        int, int *, int *, T *, hipStream_t);                 \

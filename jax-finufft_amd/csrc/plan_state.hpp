@@ -19,6 +19,8 @@ struct jfa_plan_s {
     int method = JFA_METHOD_SM;
     bool method_by_density = false;  // auto method: LDS tiles or global atomics, chosen
                                      // at setpts from the point density
+    bool keyed = false;              // sorted points without a gather: d_key_out + d_t
+                                     // (point-major), see SortedPts
     int maxsub = 4096;
     hipStream_t stream = nullptr;
     int device = 0;

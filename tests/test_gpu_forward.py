@@ -274,3 +274,27 @@ def test_upsampfac_125_all_types(ndim, prec):
     xs = [(rng.uniform(-4.0, 4.0, M)).astype(NPR[prec]) for _ in range(ndim)]
     got3 = host(nufft3(dev(c), *[dev(a) for a in xs], *[dev(a) for a in s], eps=eps, opts=opts))
     assert rel_l2(got3, oracle.type3(c, xs, s, -1)) <= eps
+
+
+@pytest.mark.parametrize("ndim,prec", product([1, 2], ["f32", "f64"]))
+@pytest.mark.parametrize("method", [1, 2])
+def test_keyed_and_gathered_sorted_points(ndim, prec, method):
+    # power-of-two bins carry the first taps in the sort key and read the
+    # Horner variables by original index; other bin sizes gather the sorted
+    # points.  Both layouts, both sorted methods, against the direct sum.
+    rng = np.random.default_rng(40 + ndim)
+    M = 4000
+    n_modes = (1500,) if ndim == 1 else (40, 52)
+    odd = dict(gpu_binsizex=1000) if ndim == 1 else dict(gpu_binsizex=24, gpu_binsizey=20)
+    eps = 1e-5 if prec == "f32" else 1e-9
+    x = clustered_points(rng, ndim, M, prec)
+    c = cnormal(rng, M, prec)
+    f = cnormal(rng, n_modes, prec)
+    ref1 = oracle.type1(x, c, n_modes, 1)
+    ref2 = oracle.type2(f, x, -1)
+    for extra in ({}, odd):
+        opts = Opts(gpu_method=method, **extra)
+        got1 = host(nufft1(n_modes, dev(c), *[dev(a) for a in x], eps=eps, opts=opts))
+        got2 = host(nufft2(dev(f), *[dev(a) for a in x], eps=eps, opts=opts))
+        assert rel_l2(got1, ref1) <= eps, (extra, rel_l2(got1, ref1))
+        assert rel_l2(got2, ref2) <= eps, (extra, rel_l2(got2, ref2))
