@@ -88,13 +88,21 @@ struct StripGeom {
     int H, K;       // rows per unit, units per strip
     int nunits;     // S * K
     int ns;
-    int pmax;       // points per piece of a crowded unit
+    int pmax;       // most points per piece (a unit has ceil(count / pmax) pieces)
 };
 
+// Complex cells of one piece's scratch tile, and the most pieces M points can
+// make (sum over units of ceil(count / pmax) <= units + M / pmax).
+inline int64_t strip_tile_cells(const StripGeom &sg) { return (int64_t)(sg.H + sg.ns - 1) * 64; }
+inline int64_t strip_max_pieces(const StripGeom &sg, int64_t M) {
+    return sg.nunits + M / sg.pmax + 1;
+}
+
 // Strip geometry for a 2D grid g (nf and ns set): units of H rows sized for
-// about 4096 units, H in [ns, nf1 - ns + 1] so that a cell is covered by at
-// most two units per axis; also sets g's bins to (strip, row), rows fastest.
-StripGeom strip_geometry(GridGeom &g, int pmax);
+// about 4096 units, at least min_h rows, H in [ns, nf1 - ns + 1] so that a
+// cell is covered by at most two units per axis; also sets g's bins to
+// (strip, row), rows fastest.
+StripGeom strip_geometry(GridGeom &g, int pmax, int min_h);
 int ensure_rocfft();
 
 // Ragged batch of 2D type-1 problems (ragged.hip).  Per problem: its grid and

@@ -25,12 +25,13 @@
 // 2*ns FMAs into its window.  When the next point starts a later row, the
 // window's first row is complete and is written out.  No atomics touch LDS.
 //
-// Output.  A unit's rows are written with plain coalesced stores into its own
-// scratch tile of (H + ns - 1) x 64 cells; a merge kernel then sums, for every
-// fine-grid cell, the (at most four) tiles that cover it.  A unit with more
-// than `pmax` points is split into pieces processed by separate waves; those
-// pieces add their rows to the (zeroed) fine grid with global atomics instead,
-// and the merge skips their tiles.
+// Output.  A unit's points are cut into pieces of at most `pmax` points (an
+// empty unit has none), one wave per piece.  A piece writes its rows with plain
+// coalesced stores into its own scratch tile of (H + ns - 1) x 64 cells; a
+// merge kernel then writes every fine-grid cell once, as the sum over the (at
+// most four) units covering it of their pieces' tiles.  The fine grid is
+// neither zeroed first nor read, no float atomics are used anywhere, and the
+// summation order is fixed, so results are bitwise reproducible.
 //
 // The same per-piece and per-cell bodies serve one plan (a single problem,
 // blockIdx.y = transform) and a ragged batch of problems (descriptor tables,
@@ -99,7 +100,7 @@ struct PieceRange {
 };
 __device__ __forceinline__ PieceRange piece_range(const StripGeom &sg,
                                                   const int *__restrict__ bin_start, int u,
-                                                  int sub, bool heavy) {
+                                                  int sub) {
     const int s = u / sg.K, k = u - s * sg.K;
     PieceRange pr;
     pr.ox = s * sg.SW;
@@ -107,8 +108,8 @@ __device__ __forceinline__ PieceRange piece_range(const StripGeom &sg,
     const int r1 = min(pr.r0 + sg.H, (int)sg.nf1);
     const int ub = bin_start[(int64_t)s * sg.nf1 + pr.r0];
     const int ue = bin_start[(int64_t)s * sg.nf1 + r1];
-    pr.pb = heavy ? ub + sub * sg.pmax : ub;
-    pr.pe = heavy ? min(pr.pb + sg.pmax, ue) : ue;
+    pr.pb = ub + sub * sg.pmax;
+    pr.pe = min(pr.pb + sg.pmax, ue);
     return pr;
 }
 
@@ -117,32 +118,21 @@ __device__ __forceinline__ PieceRange piece_range(const StripGeom &sg,
 template <typename T, int NS>
 __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T> &pts,
                                            const int *__restrict__ bin_start, int u, int sub,
-                                           bool heavy, const T *__restrict__ coef, T beta,
-                                           int horner, T *__restrict__ fw,
-                                           T *__restrict__ scratch, StripLds<T, NS> &lds) {
+                                           const T *__restrict__ coef, T beta, int horner,
+                                           T *__restrict__ tile, StripLds<T, NS> &lds) {
     const int lane = threadIdx.x;
     const int rows_tile = sg.H + NS - 1;
-    const PieceRange pr = piece_range(sg, bin_start, u, sub, heavy);
-    const int ox = pr.ox, r0 = pr.r0, pb = pr.pb, pe = pr.pe;
+    const PieceRange pr = piece_range(sg, bin_start, u, sub);
+    const int r0 = pr.r0, pb = pr.pb, pe = pr.pe;
     // key >> kStripKeyShift = rowbase + l0y for every point of this strip
     const int64_t rowbase = pts.bin_off + (int64_t)(u / sg.K) * sg.nf1;
     T acc_r[NS], acc_i[NS];
 #pragma unroll
     for (int b = 0; b < NS; ++b) acc_r[b] = acc_i[b] = T(0);
     // rolling window: acc[b] holds row rcur + b
-    int rcur = heavy ? (pe > pb ? (int)((pts.key[pb] >> kStripKeyShift) - rowbase) : r0) : r0;
-    T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
+    int rcur = r0;
     auto flush = [&](int row) {
-        if (heavy) {
-            int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
-            int64_t x = ox + lane;
-            if (x >= sg.nf0) x -= sg.nf0;
-            const int64_t gi = y * sg.nf0 + x;
-            if (acc_r[0] != T(0) || acc_i[0] != T(0)) {
-                atomic_add_global(&fw[2 * gi], acc_r[0]);
-                atomic_add_global(&fw[2 * gi + 1], acc_i[0]);
-            }
-        } else {
+        {
             const int64_t q = (int64_t)(row - r0) * kStripLanes + lane;
             tile[2 * q] = acc_r[0];
             tile[2 * q + 1] = acc_i[0];
@@ -200,8 +190,8 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
         }
         __syncthreads();
     }
-    // drain the window (and, for a full unit, every remaining row of its tile)
-    const int rstop = heavy ? rcur + NS : r0 + rows_tile;
+    // drain the window and every remaining row of the tile
+    const int rstop = r0 + rows_tile;
     while (rcur < rstop) {
         flush(rcur);
         ++rcur;
@@ -244,40 +234,26 @@ __device__ __forceinline__ void strip8_walk(const Pair<T> *__restrict__ s_w,
 template <typename T, int NS>
 __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts<T> &pts,
                                              const int *__restrict__ bin_start, int u, int sub,
-                                             bool heavy, const T *__restrict__ coef, T beta,
-                                             int horner, T *__restrict__ fw,
-                                             T *__restrict__ scratch, StripLds<T, NS> &lds) {
+                                             const T *__restrict__ coef, T beta, int horner,
+                                             T *__restrict__ tile, StripLds<T, NS> &lds) {
     static_assert(NS <= 8, "modular ownership needs ns <= 8");
     Pair<T> *s_w = lds.w;
     Pair<T> *s_cky = lds.cky;
     const int lane = threadIdx.x;
     const int a = lane & 7, r = lane >> 3;
     const int rows_tile = sg.H + NS - 1;
-    const PieceRange pr = piece_range(sg, bin_start, u, sub, heavy);
-    const int ox = pr.ox, r0 = pr.r0, pb = pr.pb, pe = pr.pe;
+    const PieceRange pr = piece_range(sg, bin_start, u, sub);
+    const int r0 = pr.r0, pb = pr.pb, pe = pr.pe;
     // key >> kStripKeyShift = rowbase + l0y for every point of this strip
     const int64_t rowbase = pts.bin_off + (int64_t)(u / sg.K) * sg.nf1;
     T acc_r[8], acc_i[8];
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
     // the window holds rows R .. R+7; row y sits in the lanes with r == y & 7
-    int R = heavy ? (pe > pb ? (int)((pts.key[pb] >> kStripKeyShift) - rowbase) : r0) : r0;
-    T *tile = scratch + 2 * (int64_t)u * rows_tile * kStripLanes;
+    int R = r0;
     auto flush = [&](int row) {
         if ((row & 7) == r) {
-            if (heavy) {
-                const int64_t y = row >= sg.nf1 ? row - sg.nf1 : row;
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    int64_t x = ox + 8 * b + a;
-                    if (x >= sg.nf0) x -= sg.nf0;
-                    const int64_t gi = y * sg.nf0 + x;
-                    if (acc_r[b] != T(0) || acc_i[b] != T(0)) {
-                        atomic_add_global(&fw[2 * gi], acc_r[b]);
-                        atomic_add_global(&fw[2 * gi + 1], acc_i[b]);
-                    }
-                }
-            } else {
+            {
                 Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile) +
                                (int64_t)(row - r0) * kStripLanes + a;
 #pragma unroll
@@ -353,8 +329,8 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
         }
         __syncthreads();
     }
-    // drain the window (and, for a full unit, every remaining row of its tile)
-    const int rstop = heavy ? R + NS : r0 + rows_tile;
+    // drain the window and every remaining row of the tile
+    const int rstop = r0 + rows_tile;
     while (R < rstop) {
         flush(R);
         ++R;
@@ -364,29 +340,32 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
 template <typename T, int NS>
 __device__ __forceinline__ void strip_piece(const StripGeom &sg, const StripPts<T> &pts,
                                             const int *__restrict__ bin_start, int u, int sub,
-                                            bool heavy, const T *__restrict__ coef, T beta,
-                                            int horner, T *__restrict__ fw,
-                                            T *__restrict__ scratch, StripLds<T, NS> &lds) {
+                                            const T *__restrict__ coef, T beta, int horner,
+                                            T *__restrict__ tile, StripLds<T, NS> &lds) {
     if constexpr (NS <= 8)
-        strip8_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, fw, scratch,
-                            lds);
+        strip8_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner, tile, lds);
     else
-        wide_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, fw, scratch,
-                          lds);
+        wide_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner, tile, lds);
 }
 
-// fw[q] += sum of the scratch tiles of the light units covering cell q.
+// Cell q of one problem's fine grid: the sum of the tiles of every piece of
+// every unit covering it.  piece_start holds the problem's units' first piece
+// numbers; tile i of `scratch` belongs to piece piece_start[0] + i.
 template <typename T, int NS>
 __device__ __forceinline__ void merge_cell(const StripGeom &sg, const int *__restrict__ piece_start,
                                            const T *__restrict__ scratch, T *__restrict__ fw,
                                            int64_t q) {
-    const int x = (int)(q % sg.nf0), y = (int)(q / sg.nf0);
+    // fine grids hold at most 2^32 cells
+    const unsigned yq = (unsigned)q / (unsigned)sg.nf0;
+    const int x = (int)((unsigned)q - yq * (unsigned)sg.nf0), y = (int)yq;
     const int rows_tile = sg.H + NS - 1;
+    const int64_t tile_cells = (int64_t)rows_tile * kStripLanes;
+    const int base = piece_start[0];
     // candidate strips: the one holding x, the previous one, and the last two
     // (whose columns wrap past nf0)
     int sc[4] = {x / sg.SW, x / sg.SW - 1, sg.S - 1, sg.S - 2};
     int kc[4] = {y / sg.H, y / sg.H - 1, sg.K - 1, sg.K - 2};
-    T ar = fw[2 * q], ai = fw[2 * q + 1];
+    T ar = T(0), ai = T(0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int s = sc[i];
@@ -406,23 +385,25 @@ __device__ __forceinline__ void merge_cell(const StripGeom &sg, const int *__res
             if (row < 0) row += (int)sg.nf1;
             if (row >= rows_tile) continue;
             const int u = s * sg.K + k;
-            if (piece_start[u + 1] - piece_start[u] != 1) continue;  // heavy: already added
-            const int64_t t = ((int64_t)u * rows_tile + row) * kStripLanes + col;
-            ar += scratch[2 * t];
-            ai += scratch[2 * t + 1];
+            const int64_t off = (int64_t)row * kStripLanes + col;
+            for (int pc = piece_start[u], pend = piece_start[u + 1]; pc < pend; ++pc) {
+                const int64_t t = (pc - base) * tile_cells + off;
+                ar += scratch[2 * t];
+                ai += scratch[2 * t + 1];
+            }
         }
     }
     fw[2 * q] = ar;
     fw[2 * q + 1] = ai;
 }
 
-// pieces of unit u: 1, or ceil(count / pmax) for a crowded unit
+// pieces of unit u: ceil(count / pmax), none for an empty unit
 __device__ __forceinline__ int unit_pieces(const StripGeom &sg, const int *__restrict__ bin_start,
                                            int u) {
     const int s = u / sg.K, k = u - s * sg.K;
     const int r0 = k * sg.H, r1 = min(r0 + sg.H, (int)sg.nf1);
     const int cnt = bin_start[(int64_t)s * sg.nf1 + r1] - bin_start[(int64_t)s * sg.nf1 + r0];
-    return cnt <= sg.pmax ? 1 : (cnt + sg.pmax - 1) / sg.pmax;
+    return (cnt + sg.pmax - 1) / sg.pmax;
 }
 
 // One workgroup: exclusive prefix of pieces(u) over u in [0, n); total at
@@ -463,19 +444,17 @@ __global__ void __launch_bounds__(kStripLanes)
     spread_strip_kernel(StripGeom sg, StripPts<T> pts, const int *__restrict__ bin_start,
                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
                         const T *__restrict__ coef, T beta, int horner, int64_t pack_stride,
-                        T *__restrict__ fw, int64_t fw_stride, T *__restrict__ scratch,
-                        int64_t scratch_stride) {
+                        T *__restrict__ scratch, int64_t scratch_stride) {
     __shared__ StripLds<T, NS> lds;
     pts.pack += 4 * pack_stride * blockIdx.y;
-    fw += 2 * fw_stride * blockIdx.y;
     scratch += 2 * scratch_stride * blockIdx.y;
+    const int64_t tile_cells = (int64_t)(sg.H + NS - 1) * kStripLanes;
     const int total = *npieces;
     for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
         const int u = segment_of(piece_start, sg.nunits, piece);
         const int sub = piece - piece_start[u];
-        const bool heavy = piece_start[u + 1] - piece_start[u] > 1;
-        strip_piece<T, NS>(sg, pts, bin_start, u, sub, heavy, coef, beta, horner, fw, scratch,
-                           lds);
+        strip_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner,
+                           scratch + 2 * piece * tile_cells, lds);
     }
 }
 
@@ -586,7 +565,7 @@ __global__ void __launch_bounds__(kStripLanes)
                         int nv, StripPts<T> pts, const int *__restrict__ bin_start,
                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
                         const T *__restrict__ coef, T beta, int horner,
-                        const void *const *__restrict__ ptrs, int nprob, T *__restrict__ fw_all,
+                        const void *const *__restrict__ ptrs, int nprob,
                         T *__restrict__ scratch_all) {
     __shared__ StripLds<T, NS> lds;
     const int total = *npieces;
@@ -596,14 +575,14 @@ __global__ void __launch_bounds__(kStripLanes)
         const int v = segment_of(vunit_off, nv, gu);
         const RaggedVirt d = virt[v];
         const int sub = piece - piece_start[gu];
-        const bool heavy = piece_start[gu + 1] - piece_start[gu] > 1;
         StripPts<T> vp = pts;
         vp.c = (const T *)ptrs[2 * nprob + d.prob] + 2 * (int64_t)d.t * d.m;
         vp.bin_off = d.bin_off;
         vp.idx_off = d.moff;
-        strip_piece<T, NS>(d.sg, vp, bin_start + d.bin_off, gu - d.unit_off, sub, heavy, coef,
-                           beta, horner, fw_all + 2 * d.fw_off, scratch_all + 2 * d.tile_off,
-                           lds);
+        const int64_t tile_cells = (int64_t)(d.sg.H + NS - 1) * kStripLanes;
+        T *tile = scratch_all + 2 * (d.tile_off + (piece - piece_start[d.unit_off]) * tile_cells);
+        strip_piece<T, NS>(d.sg, vp, bin_start + d.bin_off, gu - d.unit_off, sub, coef, beta,
+                           horner, tile, lds);
     }
 }
 
@@ -641,8 +620,8 @@ int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *
     JFA_DISPATCH_NS(sg.ns, {
         if (phases & kStripSpread)
             spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
-                sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, pack_stride, fw,
-                fw_stride, scratch, scratch_stride);
+                sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, pack_stride,
+                scratch, scratch_stride);
         if (phases & kStripMerge)
             strip_merge_kernel<T, NS_>
                 <<<dim3((unsigned)((F + 255) / 256), ntr), 256, 0, stream>>>(
@@ -656,7 +635,7 @@ template <typename T>
 int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T *coef,
                         int kerevalmeth, double beta, T *fw_all, T *scratch_all, int phases,
                         hipStream_t stream) {
-    // no points anywhere: the zeroed fine grids are already the result
+    // no points anywhere: the caller zeroes the fine grids instead
     if (rl.nv == 0 || pts.M == 0) return JFA_OK;
     if (phases & kStripUnits) {
         const unsigned nblk = (unsigned)((rl.nunits + 255) / 256);
@@ -678,7 +657,7 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
         if (phases & kStripSpread)
             ragged_strip_kernel<T, NS_><<<grid, kStripLanes, 0, stream>>>(
                 rl.virt, rl.vunit_off, rl.nv, pts, rl.bin_start, rl.piece_start, rl.npieces, coef,
-                b, kerevalmeth, rl.ptrs, rl.nprob, fw_all, scratch_all);
+                b, kerevalmeth, rl.ptrs, rl.nprob, scratch_all);
         if (phases & kStripMerge)
             ragged_merge_kernel<T, NS_><<<(unsigned)rl.cell_blocks, 256, 0, stream>>>(
                 rl.virt, rl.cell_blk, rl.nv, rl.piece_start, scratch_all, fw_all);

@@ -162,7 +162,7 @@ bool jfa::strip_applicable(jfa_plan p) {
 // Strip geometry and bins for the 2D strip spreader, and its buffers.  Units
 // of H rows are sized for about 4096 units (a few waves per SIMD), with H in
 // [ns, nf1 - ns + 1] so that a cell is covered by at most two units per axis.
-jfa::StripGeom jfa::strip_geometry(GridGeom &g, int pmax) {
+jfa::StripGeom jfa::strip_geometry(GridGeom &g, int pmax, int min_h) {
     StripGeom sg{};
     const int ns = g.ns;
     sg.nf0 = g.nf[0];
@@ -171,7 +171,7 @@ jfa::StripGeom jfa::strip_geometry(GridGeom &g, int pmax) {
     sg.SW = 64 - ns + 1;
     sg.S = (int)((g.nf[0] + sg.SW - 1) / sg.SW);
     int64_t H = (g.nf[1] * sg.S + 4095) / 4096;
-    if (H < 16) H = 16;
+    if (H < min_h) H = min_h;
     if (H > g.nf[1] - ns + 1) H = g.nf[1] - ns + 1;
     if (H < ns) H = ns;
     sg.H = (int)H;
@@ -195,14 +195,19 @@ jfa::StripGeom jfa::strip_geometry(GridGeom &g, int pmax) {
 int jfa::setup_strip(jfa_plan p) {
     GridGeom &g = p->geom;
     StripGeom &sg = p->sg;
-    const int ns = g.ns;
-    sg = strip_geometry(g, p->opts.gpu_maxsubprobsize);
-    // buffers
+    // a single problem fills the GPU with short units
+    sg = strip_geometry(g, p->opts.gpu_maxsubprobsize, 8);
+    // piece table; the tiles are sized by the point count (reserve_strip_scratch)
     free_ptr(reinterpret_cast<void *&>(p->d_piece_start));
     JFA_HIP_CHECK(hipMalloc(&p->d_piece_start, sizeof(int) * (sg.nunits + 1)));
     if (!p->d_npieces) JFA_HIP_CHECK(hipMalloc(&p->d_npieces, sizeof(int) * 4));
-    const int64_t rows_tile = sg.H + ns - 1;
-    p->strip_scratch_stride = (int64_t)sg.nunits * rows_tile * 64;
+    return JFA_OK;
+}
+
+// One scratch tile per piece of M points, for each transform of a batch.
+static int reserve_strip_scratch(jfa_plan p, int64_t M) {
+    const StripGeom &sg = p->sg;
+    p->strip_scratch_stride = strip_max_pieces(sg, M) * strip_tile_cells(sg);
     const size_t bytes =
         (size_t)p->batch * p->strip_scratch_stride * 2 * (p->is_double ? 8 : 4);
     if (bytes > p->strip_scratch_bytes) {
@@ -702,7 +707,8 @@ BinIndex jfa::bin_view(jfa_plan p) {
 }
 
 // Spread ntr transforms (strengths c with stride c_stride, grids fw with
-// stride fw_stride, fw already zeroed) with the plan's method.
+// stride fw_stride) with the plan's method.  fw must be zeroed first except
+// for the strip method, which writes every cell.
 // With a stage timer, the strip method's three launches are marked as the
 // stages "units", "spread" (the spreading kernel alone) and "merge".
 template <typename T>
@@ -716,9 +722,12 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
         if (tm) tm->mark("spread");
         return rc;
     }
+    // the merge writes every cell of fw, unless there are no points at all
+    const int64_t M = p->M;
+    if (M == 0) return launch_fill_zero(fw, (int64_t)sizeof(T) * 2 * fw_stride * ntr, p->stream);
+    if (int rc = reserve_strip_scratch(p, M)) return rc;
     // one record (c, tx, ty) per point and transform of a batch: the spreader's
     // only scattered read
-    const int64_t M = p->M;
     const size_t pack_bytes = (size_t)std::min(p->batch, ntr) * M * 4 * sizeof(T);
     if (pack_bytes > p->pack_bytes) {
         free_ptr(p->d_pack);
@@ -775,9 +784,11 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
     if (p->opts.gpu_spreadinterponly) {
         // the mode array is the fine grid itself
         if (p->type == 1) {
-            if ((rc = launch_fill_zero(fk, sizeof(T) * 2 * p->F * p->ntransf, p->stream)))
-                return rc;
-            tm.mark("zero");
+            if (p->method != JFA_METHOD_STRIP) {
+                if ((rc = launch_fill_zero(fk, sizeof(T) * 2 * p->F * p->ntransf, p->stream)))
+                    return rc;
+                tm.mark("zero");
+            }
             rc = spread_timed<T>(p, c, M, fk, p->F, p->ntransf, tm.on ? &tm : nullptr);
         } else {
             rc = launch_interp<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, fk, p->F, c, M,
@@ -793,8 +804,10 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
         T *cb = c + 2 * (int64_t)t0 * M;
         T *fb = fk + 2 * (int64_t)t0 * p->Ntot;
         if (p->type == 1) {
-            if ((rc = launch_fill_zero(fw, sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
-            tm.mark("zero");
+            if (p->method != JFA_METHOD_STRIP) {
+                if ((rc = launch_fill_zero(fw, sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
+                tm.mark("zero");
+            }
             if ((rc = spread_timed<T>(p, cb, M, fw, p->F, nb, tm.on ? &tm : nullptr))) return rc;
             if ((rc = fft_exec(p, fw, nb))) return rc;
             tm.mark("fft");

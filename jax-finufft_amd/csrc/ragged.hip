@@ -168,7 +168,8 @@ int layout(jfa_ragged r) {
                       (long long)kMaxFineCells);
             return JFA_ERR_TOO_LARGE;
         }
-        sg[i] = strip_geometry(g, r->opts.gpu_maxsubprobsize);
+        // many problems fill the GPU together: taller units, less tile overlap
+        sg[i] = strip_geometry(g, r->opts.gpu_maxsubprobsize, 16);
         r->probs[i].moff = r->moff[i];
         r->probs[i].bin_off = bins;
         r->moff[i + 1] = r->moff[i] + r->npts[i];
@@ -224,7 +225,7 @@ int layout(jfa_ragged r) {
             v.moff = r->moff[i];
             r->virt.push_back(v);
             fw += F;
-            tiles += (int64_t)sg[i].nunits * (sg[i].H + ns - 1) * 64;
+            tiles += strip_max_pieces(sg[i], r->npts[i]) * strip_tile_cells(sg[i]);
             r->vunit_off.push_back(r->vunit_off.back() + sg[i].nunits);
             r->cell_blk.push_back(r->cell_blk.back() + (F + 255) / 256);
             r->mode_blk.push_back(r->mode_blk.back() + (v.n0 * v.n1 + 255) / 256);
@@ -368,11 +369,14 @@ StripPts<T> sorted(jfa_ragged r) {
                        0, 0};
 }
 
-// Everything after the sort: zero the fine grids, spread, merge, FFT, deconvolve.
+// Everything after the sort: spread, merge, FFT, deconvolve.
 template <typename T>
 int enqueue_execute(jfa_ragged r, hipStream_t s) {
     const int kem = r->opts.gpu_kerevalmeth ? 1 : 0;
-    if (int rc = launch_fill_zero(r->d_fw, (int64_t)(2 * sizeof(T)) * r->fw_cells, s)) return rc;
+    // the merge writes every fine-grid cell; with no points at all it does not run
+    if (r->mtot == 0)
+        if (int rc = launch_fill_zero(r->d_fw, (int64_t)(2 * sizeof(T)) * r->fw_cells, s))
+            return rc;
     if (int rc = launch_ragged_strip<T>(r->rl, sorted<T>(r), (const T *)r->d_coef, kem,
                                         r->kp.beta, (T *)r->d_fw, (T *)r->d_scratch, kStripAll,
                                         s))

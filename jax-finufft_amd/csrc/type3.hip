@@ -295,7 +295,10 @@ static int type3_execute_t(jfa_plan p, const T *c, T *f) {
                 return rc;
             src = (const T *)p->d_cp;
         }
-        if ((rc = launch_fill_zero(fw, (int64_t)sizeof(T) * 2 * p->F * nb, p->stream))) return rc;
+        // the strip spreader writes every cell itself
+        if (p->method != JFA_METHOD_STRIP)
+            if ((rc = launch_fill_zero(fw, (int64_t)sizeof(T) * 2 * p->F * nb, p->stream)))
+                return rc;
         if ((rc = spread_any<T>(p, src, M, fw, p->F, nb))) return rc;
         const int saved = p->inner->ntransf;
         p->inner->ntransf = nb;
