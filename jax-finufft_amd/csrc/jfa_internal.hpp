@@ -132,13 +132,14 @@ struct RaggedLaunch {
     const RaggedVirt *virt;
     const int64_t *moff;       // nprob + 1
     const int *vunit_off;      // nv + 1
-    const int64_t *cell_blk;   // nv + 1: first merge block (256 cells) per virtual problem
     const int64_t *mode_blk;   // nv + 1: first deconv block (256 modes)
     const void *const *ptrs;   // x[nprob], y[nprob], c[nprob], f[nprob]
+    const int *unit_virt;      // nunits: virtual problem of every global unit
     int *bin_start, *piece_start, *npieces;
+    int *piece_unit;           // global unit of every piece
     int *blocksum;             // 2 * ceil(nunits / 256) + 1 ints of piece-scan scratch
     int nprob, nv, ns, pmax, keyshift;
-    int64_t mtot, nbins, nunits, cell_blocks, mode_blocks;
+    int64_t mtot, nbins, nunits, mode_blocks;
 };
 
 // Points after setpts, in bin-sorted order: original index, first tap
@@ -228,10 +229,10 @@ int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const 
                   T *fw, int64_t fw_stride, int ntr, hipStream_t stream);
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
-                        int *piece_start, int *npieces, const T *coef, int kerevalmeth,
-                        double beta, int64_t pack_stride, T *fw, int64_t fw_stride,
-                        T *scratch, int64_t scratch_stride, int ntr, int phases,
-                        hipStream_t stream);
+                        int *piece_start, int *npieces, int *piece_unit, const T *coef,
+                        int kerevalmeth, double beta, int64_t pack_stride, T *fw,
+                        int64_t fw_stride, T *scratch, int64_t scratch_stride, int ntr,
+                        int phases, hipStream_t stream);
 // pack[t][j] = (c[t][j], rec[j]) for t < ntr, j < M (records of 4 reals)
 template <typename T>
 int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int ntr, T *pack,

@@ -255,21 +255,30 @@ __device__ __forceinline__ int problem_of(const int64_t *__restrict__ moff, int 
     return lo;
 }
 
+// kRaggedKeysPer points per thread (strided by the workgroup size), so that the
+// problem lookup is paid once per thread rather than once per point
+constexpr int kRaggedKeysPer = 8;
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     ragged_keys_kernel(const RaggedProb *__restrict__ probs, const int64_t *__restrict__ moff,
                        int nprob, const void *const *__restrict__ ptrs, int64_t mtot,
                        unsigned *key, unsigned *val, T *rec) {
-    const int64_t P = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (P >= mtot) return;
-    const int i = problem_of(moff, nprob, P);
-    const RaggedProb &pr = probs[i];
-    T tx, ty;
-    key[P] = strip_key<T>(pr.g, pr.bin_off, (const T *)ptrs[i], (const T *)ptrs[nprob + i],
-                          P - pr.moff, tx, ty);
-    val[P] = (unsigned)P;
-    rec[2 * P] = tx;
-    rec[2 * P + 1] = ty;
+    const int64_t P0 = (int64_t)blockIdx.x * 256 * kRaggedKeysPer + threadIdx.x;
+    if (P0 >= mtot) return;
+    int i = problem_of(moff, nprob, P0);
+    for (int r = 0; r < kRaggedKeysPer; ++r) {
+        const int64_t P = P0 + (int64_t)r * 256;
+        if (P >= mtot) break;
+        while (P >= moff[i + 1]) ++i;  // points ascend: the next problems
+        const RaggedProb &pr = probs[i];
+        T tx, ty;
+        key[P] = strip_key<T>(pr.g, pr.bin_off, (const T *)ptrs[i], (const T *)ptrs[nprob + i],
+                              P - pr.moff, tx, ty);
+        val[P] = (unsigned)P;
+        rec[2 * P] = tx;
+        rec[2 * P + 1] = ty;
+    }
 }
 
 __global__ void fill_zero_kernel(uint4 *p, int64_t n16, unsigned char *tail, int ntail) {
@@ -398,12 +407,17 @@ int launch_fold_points(const GridGeom &g, int64_t M, const T *x, const T *y, con
 }
 
 // Shared tail of the strip sorts: radix sort of (key, index) and bin bounds.
+// The strip spreaders need the points of a strip in row order and, within a
+// row, grouped by 8-column block (X >> 3); the order within a block does not
+// matter, so the three lowest key bits are left out of the (stable) sort.
+constexpr int kStripSortLowBit = 3;
+
 int sort_strip_keys(int64_t M, int64_t nbins, const SortScratch &s, int *bin_start, int *idx,
                     hipStream_t stream) {
     if (M > 0) {
         size_t bytes = s.temp_bytes;
         JFA_HIP_CHECK(rocprim::radix_sort_pairs(s.temp, bytes, s.key_in, s.key_out, s.val_in,
-                                                (unsigned *)idx, (unsigned)M, 0,
+                                                (unsigned *)idx, (unsigned)M, kStripSortLowBit,
                                                 key_bits(nbins) + kStripKeyShift, stream));
     }
     const unsigned blocks = (unsigned)((nbins + 1 + 255) / 256);
@@ -430,7 +444,8 @@ int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T
                        hipStream_t stream) {
     const int64_t M = rl.mtot;
     if (M > 0) {
-        const unsigned blocks = (unsigned)((M + 255) / 256);
+        const int64_t per = 256 * kRaggedKeysPer;
+        const unsigned blocks = (unsigned)((M + per - 1) / per);
         ragged_keys_kernel<T><<<blocks, 256, 0, stream>>>(rl.probs, rl.moff, rl.nprob, rl.ptrs, M,
                                                           s.key_in, s.val_in, rec);
         JFA_HIP_CHECK(hipGetLastError());

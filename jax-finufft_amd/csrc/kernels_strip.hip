@@ -28,8 +28,9 @@
 // Output.  A unit's points are cut into pieces of at most `pmax` points (an
 // empty unit has none), one wave per piece.  A piece writes its rows with plain
 // coalesced stores into its own scratch tile of (H + ns - 1) x 64 cells; a
-// merge kernel then writes every fine-grid cell once, as the sum over the (at
-// most four) units covering it of their pieces' tiles.  The fine grid is
+// merge kernel then writes every fine-grid cell once, as the sum over the
+// units covering it (one to four, up to nine next to a narrow last strip or
+// unit) of their pieces' tiles.  The fine grid is
 // neither zeroed first nor read, no float atomics are used anywhere, and the
 // summation order is fixed, so results are bitwise reproducible.
 //
@@ -42,20 +43,6 @@ namespace jfa {
 namespace {
 
 constexpr int kStripLanes = 64;
-
-// first index i in [0, n) with starts[i] <= v < starts[i + 1]
-template <typename I>
-__device__ __forceinline__ int segment_of(const I *__restrict__ starts, int n, int64_t v) {
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if ((int64_t)starts[mid] <= v)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
 
 template <typename T>
 struct alignas(2 * sizeof(T)) Pair {
@@ -348,53 +335,80 @@ __device__ __forceinline__ void strip_piece(const StripGeom &sg, const StripPts<
         wide_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner, tile, lds);
 }
 
-// Cell q of one problem's fine grid: the sum of the tiles of every piece of
-// every unit covering it.  piece_start holds the problem's units' first piece
-// numbers; tile i of `scratch` belongs to piece piece_start[0] + i.
-template <typename T, int NS>
-__device__ __forceinline__ void merge_cell(const StripGeom &sg, const int *__restrict__ piece_start,
-                                           const T *__restrict__ scratch, T *__restrict__ fw,
-                                           int64_t q) {
-    // fine grids hold at most 2^32 cells
-    const unsigned yq = (unsigned)q / (unsigned)sg.nf0;
-    const int x = (int)((unsigned)q - yq * (unsigned)sg.nf0), y = (int)yq;
-    const int rows_tile = sg.H + NS - 1;
-    const int64_t tile_cells = (int64_t)rows_tile * kStripLanes;
-    const int base = piece_start[0];
-    // candidate strips: the one holding x, the previous one, and the last two
-    // (whose columns wrap past nf0)
-    int sc[4] = {x / sg.SW, x / sg.SW - 1, sg.S - 1, sg.S - 2};
-    int kc[4] = {y / sg.H, y / sg.H - 1, sg.K - 1, sg.K - 2};
-    T ar = T(0), ai = T(0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int s = sc[i];
-        bool dup = s < 0 || s >= sg.S;
-        for (int i2 = 0; i2 < i; ++i2) dup = dup || sc[i2] == s;
-        if (dup) continue;
-        int col = x - s * sg.SW;
-        if (col < 0) col += (int)sg.nf0;
-        if (col >= kStripLanes) continue;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const int k = kc[jj];
-            bool dupk = k < 0 || k >= sg.K;
-            for (int j2 = 0; j2 < jj; ++j2) dupk = dupk || kc[j2] == k;
-            if (dupk) continue;
-            int row = y - k * sg.H;
-            if (row < 0) row += (int)sg.nf1;
-            if (row >= rows_tile) continue;
-            const int u = s * sg.K + k;
-            const int64_t off = (int64_t)row * kStripLanes + col;
-            for (int pc = piece_start[u], pend = piece_start[u + 1]; pc < pend; ++pc) {
-                const int64_t t = (pc - base) * tile_cells + off;
-                ar += scratch[2 * t];
-                ai += scratch[2 * t + 1];
+// The windows of one axis that reach into window w's first ns - 1 cells from
+// the left: window w - 1, or, for w = 0, the last one or two windows through
+// the periodic wrap (the second when the last window owns fewer than ns - 1
+// cells).  Windows start every `step` cells, span `span` and n of them cover
+// nf cells.  Appends (window, offset of w's cell 0 in its tile) to win / off.
+__device__ __forceinline__ int left_windows(int w, int step, int span, int n, int64_t nf,
+                                            int *win, int *off) {
+    int c = 0;
+    if (w > 0) {
+        win[c] = w - 1;
+        off[c] = step;
+        ++c;
+    } else {
+        for (int v = n - 1; v >= 1 && v >= n - 2; --v) {
+            const int o = (int)(nf - (int64_t)v * step);
+            if (o < span) {
+                win[c] = v;
+                off[c] = o;
+                ++c;
             }
         }
     }
-    fw[2 * q] = ar;
-    fw[2 * q + 1] = ai;
+    return c;
+}
+
+// The cells unit u owns (columns [s*SW, s*SW + SW) and rows [k*H, k*H + H),
+// clipped to the grid), each written once as the sum of the tiles of every
+// piece of every unit covering it: u itself, and the units to its left, above
+// and diagonally where their tiles reach into u's first ns - 1 columns / rows.
+// One workgroup per unit, lane = column, waves over rows; the candidate units
+// and their piece ranges are uniform over the workgroup.  piece_start holds the
+// problem's units' first piece numbers; tile i of `scratch` belongs to piece
+// piece_start[0] + i.
+template <typename T, int NS>
+__device__ __forceinline__ void merge_unit(const StripGeom &sg, const int *__restrict__ piece_start,
+                                           const T *__restrict__ scratch, T *__restrict__ fw,
+                                           int u) {
+    constexpr int SW = 65 - NS;
+    const int s = u / sg.K, k = u - s * sg.K;
+    const int x0 = s * SW, y0 = k * sg.H;
+    const int wcols = min(SW, (int)(sg.nf0 - x0));
+    const int wrows = min(sg.H, (int)(sg.nf1 - y0));
+    const int rows_tile = sg.H + NS - 1;
+    const int64_t tile_cells = (int64_t)rows_tile * kStripLanes;
+    const int base = piece_start[0];
+    int cs[3], co[3], ck[3], ro[3];
+    cs[0] = s;
+    co[0] = 0;
+    ck[0] = k;
+    ro[0] = 0;
+    const int ncs = 1 + left_windows(s, SW, kStripLanes, sg.S, sg.nf0, cs + 1, co + 1);
+    const int nck = 1 + left_windows(k, sg.H, rows_tile, sg.K, sg.nf1, ck + 1, ro + 1);
+    const int c = threadIdx.x & (kStripLanes - 1);
+    if (c >= wcols) return;
+    for (int r = threadIdx.x / kStripLanes; r < wrows; r += blockDim.x / kStripLanes) {
+        T ar = T(0), ai = T(0);
+        for (int i = 0; i < ncs; ++i) {
+            const int tc = c + co[i];
+            if (tc >= kStripLanes) continue;
+            for (int j = 0; j < nck; ++j) {
+                const int tr = r + ro[j];
+                if (tr >= rows_tile) continue;
+                const int u2 = cs[i] * sg.K + ck[j];
+                const int64_t off = (int64_t)tr * kStripLanes + tc;
+                for (int pc = piece_start[u2], pend = piece_start[u2 + 1]; pc < pend; ++pc) {
+                    const Pair<T> v =
+                        reinterpret_cast<const Pair<T> *>(scratch)[(pc - base) * tile_cells + off];
+                    ar += v.x;
+                    ai += v.y;
+                }
+            }
+        }
+        reinterpret_cast<Pair<T> *>(fw)[(int64_t)(y0 + r) * sg.nf0 + x0 + c] = Pair<T>{ar, ai};
+    }
 }
 
 // pieces of unit u: ceil(count / pmax), none for an empty unit
@@ -407,10 +421,10 @@ __device__ __forceinline__ int unit_pieces(const StripGeom &sg, const int *__res
 }
 
 // One workgroup: exclusive prefix of pieces(u) over u in [0, n); total at
-// piece_start[n] and in *npieces.
+// piece_start[n] and in *npieces; with piece_unit, also the unit of every piece.
 template <typename PiecesOf>
 __device__ __forceinline__ void piece_scan(int n, PiecesOf pieces, int *piece_start,
-                                           int *npieces) {
+                                           int *npieces, int *piece_unit) {
     __shared__ int sa[1024];
     const int tid = threadIdx.x;
     const int per = (n + 1023) / 1024;
@@ -429,7 +443,10 @@ __device__ __forceinline__ void piece_scan(int n, PiecesOf pieces, int *piece_st
     int e = sa[tid] - a;
     for (int u = lo; u < hi; ++u) {
         piece_start[u] = e;
-        e += pieces(u);
+        const int np = pieces(u);
+        if (piece_unit)
+            for (int i = 0; i < np; ++i) piece_unit[e + i] = u;
+        e += np;
     }
     if (tid == 1023) {
         piece_start[n] = sa[1023];
@@ -443,15 +460,16 @@ template <typename T, int NS>
 __global__ void __launch_bounds__(kStripLanes)
     spread_strip_kernel(StripGeom sg, StripPts<T> pts, const int *__restrict__ bin_start,
                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
-                        const T *__restrict__ coef, T beta, int horner, int64_t pack_stride,
-                        T *__restrict__ scratch, int64_t scratch_stride) {
+                        const int *__restrict__ piece_unit, const T *__restrict__ coef, T beta,
+                        int horner, int64_t pack_stride, T *__restrict__ scratch,
+                        int64_t scratch_stride) {
     __shared__ StripLds<T, NS> lds;
     pts.pack += 4 * pack_stride * blockIdx.y;
     scratch += 2 * scratch_stride * blockIdx.y;
     const int64_t tile_cells = (int64_t)(sg.H + NS - 1) * kStripLanes;
     const int total = *npieces;
     for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
-        const int u = segment_of(piece_start, sg.nunits, piece);
+        const int u = piece_unit[piece];
         const int sub = piece - piece_start[u];
         strip_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner,
                            scratch + 2 * piece * tile_cells, lds);
@@ -475,17 +493,16 @@ __global__ void __launch_bounds__(256)
     strip_merge_kernel(StripGeom sg, const int *__restrict__ piece_start,
                        const T *__restrict__ scratch, int64_t scratch_stride, T *__restrict__ fw,
                        int64_t fw_stride) {
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= sg.nf0 * sg.nf1) return;
-    merge_cell<T, NS>(sg, piece_start, scratch + 2 * scratch_stride * blockIdx.y,
-                      fw + 2 * fw_stride * blockIdx.y, q);
+    merge_unit<T, NS>(sg, piece_start, scratch + 2 * scratch_stride * blockIdx.y,
+                      fw + 2 * fw_stride * blockIdx.y, (int)blockIdx.x);
 }
 
 __global__ void __launch_bounds__(1024)
     strip_units_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
-                       int *npieces) {
+                       int *npieces, int *piece_unit) {
     piece_scan(
-        sg.nunits, [&](int u) { return unit_pieces(sg, bin_start, u); }, piece_start, npieces);
+        sg.nunits, [&](int u) { return unit_pieces(sg, bin_start, u); }, piece_start, npieces,
+        piece_unit);
 }
 
 // ------------------------------------------------------------ ragged batch
@@ -499,9 +516,9 @@ __global__ void __launch_bounds__(1024)
 // Piece table of every virtual problem's units in three launches: counts and
 // per-block sums, a scan of the block sums, then the block-local scans.
 __device__ __forceinline__ int ragged_unit_pieces(const RaggedVirt *__restrict__ virt,
-                                                  const int *__restrict__ vunit_off, int nv,
+                                                  const int *__restrict__ unit_virt,
                                                   const int *__restrict__ bin_start, int gu) {
-    const int v = segment_of(vunit_off, nv, gu);
+    const int v = unit_virt[gu];
     const RaggedVirt &d = virt[v];
     return unit_pieces(d.sg, bin_start + d.bin_off, gu - d.unit_off);
 }
@@ -525,12 +542,13 @@ __device__ __forceinline__ int block_scan256(int v, int *total) {
 
 __global__ void __launch_bounds__(256)
     ragged_units_count_kernel(const RaggedVirt *__restrict__ virt,
-                              const int *__restrict__ vunit_off, int nv,
+                              const int *__restrict__ vunit_off,
+                              const int *__restrict__ unit_virt, int nv,
                               const int *__restrict__ bin_start, int *piece_start,
                               int *blocksum) {
     const int nu = vunit_off[nv];
     const int gu = blockIdx.x * 256 + threadIdx.x;
-    const int c = gu < nu ? ragged_unit_pieces(virt, vunit_off, nv, bin_start, gu) : 0;
+    const int c = gu < nu ? ragged_unit_pieces(virt, unit_virt, bin_start, gu) : 0;
     if (gu < nu) piece_start[gu] = c;
     int total;
     block_scan256(c, &total);
@@ -544,35 +562,40 @@ __global__ void __launch_bounds__(1024)
                                const int *__restrict__ blocksum, int *blockoff,
                                int *piece_start, int *npieces) {
     piece_scan(
-        nblk, [&](int b) { return blocksum[b]; }, blockoff, npieces);
+        nblk, [&](int b) { return blocksum[b]; }, blockoff, npieces, nullptr);
     if (threadIdx.x == 1023) piece_start[vunit_off[nv]] = blockoff[nblk];
 }
 
 __global__ void __launch_bounds__(256)
     ragged_units_scan_kernel(const int *__restrict__ vunit_off, int nv,
-                             const int *__restrict__ blockoff, int *piece_start) {
+                             const int *__restrict__ blockoff, int *piece_start,
+                             int *piece_unit) {
     const int nu = vunit_off[nv];
     const int gu = blockIdx.x * 256 + threadIdx.x;
     const int c = gu < nu ? piece_start[gu] : 0;
     int total;
     const int e = block_scan256(c, &total);
-    if (gu < nu) piece_start[gu] = blockoff[blockIdx.x] + e;
+    if (gu < nu) {
+        const int first = blockoff[blockIdx.x] + e;
+        piece_start[gu] = first;
+        for (int i = 0; i < c; ++i) piece_unit[first + i] = gu;
+    }
 }
 
 template <typename T, int NS>
 __global__ void __launch_bounds__(kStripLanes)
-    ragged_strip_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ vunit_off,
-                        int nv, StripPts<T> pts, const int *__restrict__ bin_start,
+    ragged_strip_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ unit_virt,
+                        StripPts<T> pts, const int *__restrict__ bin_start,
                         const int *__restrict__ piece_start, const int *__restrict__ npieces,
+                        const int *__restrict__ piece_unit,
                         const T *__restrict__ coef, T beta, int horner,
                         const void *const *__restrict__ ptrs, int nprob,
                         T *__restrict__ scratch_all) {
     __shared__ StripLds<T, NS> lds;
     const int total = *npieces;
-    const int nu = vunit_off[nv];
     for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
-        const int gu = segment_of(piece_start, nu, piece);
-        const int v = segment_of(vunit_off, nv, gu);
+        const int gu = piece_unit[piece];
+        const int v = unit_virt[gu];
         const RaggedVirt d = virt[v];
         const int sub = piece - piece_start[gu];
         StripPts<T> vp = pts;
@@ -586,45 +609,43 @@ __global__ void __launch_bounds__(kStripLanes)
     }
 }
 
-// block b merges 256 cells of the virtual problem v with cell_blk[v] <= b
+// block b merges global unit b (unit b - unit_off of its virtual problem)
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
-    ragged_merge_kernel(const RaggedVirt *__restrict__ virt, const int64_t *__restrict__ cell_blk,
-                        int nv, const int *__restrict__ piece_start,
-                        const T *__restrict__ scratch_all, T *__restrict__ fw_all) {
-    const int v = segment_of(cell_blk, nv, (int64_t)blockIdx.x);
+    ragged_merge_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ unit_virt,
+                        const int *__restrict__ piece_start, const T *__restrict__ scratch_all,
+                        T *__restrict__ fw_all) {
+    const int gu = (int)blockIdx.x;
+    const int v = unit_virt[gu];
     const RaggedVirt d = virt[v];
-    const int64_t q = ((int64_t)blockIdx.x - cell_blk[v]) * 256 + threadIdx.x;
-    if (q >= d.sg.nf0 * d.sg.nf1) return;
-    merge_cell<T, NS>(d.sg, piece_start + d.unit_off, scratch_all + 2 * d.tile_off,
-                      fw_all + 2 * d.fw_off, q);
+    merge_unit<T, NS>(d.sg, piece_start + d.unit_off, scratch_all + 2 * d.tile_off,
+                      fw_all + 2 * d.fw_off, gu - d.unit_off);
 }
 
 }  // namespace
 
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
-                        int *piece_start, int *npieces, const T *coef, int kerevalmeth,
-                        double beta, int64_t pack_stride, T *fw, int64_t fw_stride,
-                        T *scratch, int64_t scratch_stride, int ntr, int phases,
-                        hipStream_t stream) {
+                        int *piece_start, int *npieces, int *piece_unit, const T *coef,
+                        int kerevalmeth, double beta, int64_t pack_stride, T *fw,
+                        int64_t fw_stride, T *scratch, int64_t scratch_stride, int ntr,
+                        int phases, hipStream_t stream) {
     if (ntr == 0 || pts.M == 0) return JFA_OK;
     if (phases & kStripUnits) {
-        strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces);
+        strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces,
+                                                   piece_unit);
         JFA_HIP_CHECK(hipGetLastError());
     }
     const int64_t ub = sg.nunits + pts.M / sg.pmax + 1;
     const unsigned grid = (unsigned)(ub < 256 * 32 ? ub : 256 * 32);
     const T b = (T)beta;
-    const int64_t F = sg.nf0 * sg.nf1;
     JFA_DISPATCH_NS(sg.ns, {
         if (phases & kStripSpread)
             spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
-                sg, pts, bin_start, piece_start, npieces, coef, b, kerevalmeth, pack_stride,
-                scratch, scratch_stride);
+                sg, pts, bin_start, piece_start, npieces, piece_unit, coef, b, kerevalmeth,
+                pack_stride, scratch, scratch_stride);
         if (phases & kStripMerge)
-            strip_merge_kernel<T, NS_>
-                <<<dim3((unsigned)((F + 255) / 256), ntr), 256, 0, stream>>>(
+            strip_merge_kernel<T, NS_><<<dim3((unsigned)sg.nunits, ntr), 256, 0, stream>>>(
                     sg, piece_start, scratch, scratch_stride, fw, fw_stride);
     });
     JFA_HIP_CHECK(hipGetLastError());
@@ -639,15 +660,15 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
     if (rl.nv == 0 || pts.M == 0) return JFA_OK;
     if (phases & kStripUnits) {
         const unsigned nblk = (unsigned)((rl.nunits + 255) / 256);
-        ragged_units_count_kernel<<<nblk, 256, 0, stream>>>(rl.virt, rl.vunit_off, rl.nv,
-                                                            rl.bin_start, rl.piece_start,
+        ragged_units_count_kernel<<<nblk, 256, 0, stream>>>(rl.virt, rl.vunit_off, rl.unit_virt,
+                                                            rl.nv, rl.bin_start, rl.piece_start,
                                                             rl.blocksum);
         int *blockoff = rl.blocksum + nblk;  // nblk + 1 entries
         ragged_units_blocks_kernel<<<1, 1024, 0, stream>>>(rl.vunit_off, rl.nv, (int)nblk,
                                                            rl.blocksum, blockoff,
                                                            rl.piece_start, rl.npieces);
         ragged_units_scan_kernel<<<nblk, 256, 0, stream>>>(rl.vunit_off, rl.nv, blockoff,
-                                                           rl.piece_start);
+                                                           rl.piece_start, rl.piece_unit);
         JFA_HIP_CHECK(hipGetLastError());
     }
     const int64_t ub = rl.nunits + pts.M / rl.pmax + rl.nv;
@@ -656,11 +677,11 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
     JFA_DISPATCH_NS(rl.ns, {
         if (phases & kStripSpread)
             ragged_strip_kernel<T, NS_><<<grid, kStripLanes, 0, stream>>>(
-                rl.virt, rl.vunit_off, rl.nv, pts, rl.bin_start, rl.piece_start, rl.npieces, coef,
-                b, kerevalmeth, rl.ptrs, rl.nprob, scratch_all);
+                rl.virt, rl.unit_virt, pts, rl.bin_start, rl.piece_start, rl.npieces,
+                rl.piece_unit, coef, b, kerevalmeth, rl.ptrs, rl.nprob, scratch_all);
         if (phases & kStripMerge)
-            ragged_merge_kernel<T, NS_><<<(unsigned)rl.cell_blocks, 256, 0, stream>>>(
-                rl.virt, rl.cell_blk, rl.nv, rl.piece_start, scratch_all, fw_all);
+            ragged_merge_kernel<T, NS_><<<(unsigned)rl.nunits, 256, 0, stream>>>(
+                rl.virt, rl.unit_virt, rl.piece_start, scratch_all, fw_all);
     });
     JFA_HIP_CHECK(hipGetLastError());
     return JFA_OK;
@@ -678,8 +699,8 @@ int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int
 
 #define JFA_INST(T)                                                                           \
     template int launch_spread_strip<T>(const StripGeom &, const StripPts<T> &, const int *, \
-                                        int *, int *, const T *, int, double, int64_t, T *,   \
-                                        int64_t, T *, int64_t, int, int, hipStream_t);        \
+                                        int *, int *, int *, const T *, int, double, int64_t, \
+                                        T *, int64_t, T *, int64_t, int, int, hipStream_t);   \
     template int launch_strip_pack<T>(const T *, int64_t, const T *, int64_t, int, T *,       \
                                       hipStream_t);                                           \
     template int launch_ragged_strip<T>(const RaggedLaunch &, const StripPts<T> &, const T *, \

@@ -207,6 +207,12 @@ int jfa::setup_strip(jfa_plan p) {
 // One scratch tile per piece of M points, for each transform of a batch.
 static int reserve_strip_scratch(jfa_plan p, int64_t M) {
     const StripGeom &sg = p->sg;
+    if (strip_max_pieces(sg, M) > p->piece_unit_cap) {
+        free_ptr(reinterpret_cast<void *&>(p->d_piece_unit));
+        p->piece_unit_cap = 0;
+        JFA_HIP_CHECK(hipMalloc(&p->d_piece_unit, sizeof(int) * strip_max_pieces(sg, M)));
+        p->piece_unit_cap = strip_max_pieces(sg, M);
+    }
     p->strip_scratch_stride = strip_max_pieces(sg, M) * strip_tile_cells(sg);
     const size_t bytes =
         (size_t)p->batch * p->strip_scratch_stride * 2 * (p->is_double ? 8 : 4);
@@ -751,7 +757,8 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
         for (int ph = 0; ph < 3; ++ph) {
             if (!tm && ph > 0) break;
             int rc = launch_spread_strip<T>(
-                p->sg, pts, p->d_bin_start, p->d_piece_start, p->d_npieces, (const T *)p->d_coef,
+                p->sg, pts, p->d_bin_start, p->d_piece_start, p->d_npieces, p->d_piece_unit,
+                (const T *)p->d_coef,
                 kem, p->kp.beta, M, fw + 2 * t0 * fw_stride, fw_stride, (T *)p->d_strip_scratch,
                 p->strip_scratch_stride, nb, tm ? phases[ph] : kStripAll, p->stream);
             if (rc) return rc;
@@ -968,7 +975,8 @@ extern "C" int jfa_destroy(jfa_plan p) {
     for (void **q : {(void **)&p->d_idx, (void **)&p->d_l0, (void **)&p->d_key_in,
                      (void **)&p->d_val_in, (void **)&p->d_key_out, &p->d_sort_temp, &p->d_t,
                      (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub,
-                     (void **)&p->d_piece_start, (void **)&p->d_npieces, &p->d_strip_scratch,
+                     (void **)&p->d_piece_start, (void **)&p->d_npieces,
+                     (void **)&p->d_piece_unit, &p->d_strip_scratch,
                      &p->d_pack})
         free_ptr(*q);
     jfa::type3_free(p);

@@ -9,9 +9,9 @@
 // is its bin within its problem offset by the problem's first global bin.  A
 // virtual problem is one (problem, transform) pair; virtual problems are
 // ordered by fine-grid size so that the fine grids of one size are contiguous
-// and transform together in one batched rocFFT call.  The spreading kernel
-// walks the global list of units of every virtual problem; merge and
-// deconvolution walk the global lists of 256-cell / 256-mode blocks.  The
+// and transform together in one batched rocFFT call.  The spreading and merge
+// kernels walk the global list of units of every virtual problem;
+// deconvolution walks the global list of 256-mode blocks.  The
 // device pointer table (x, y, c, f of every problem) is rewritten before each
 // run, so a captured graph stays valid for new arrays of the same shapes.
 #include <algorithm>
@@ -34,13 +34,14 @@ struct jfa_ragged_s {
     std::vector<int64_t> npts, ntr, n0, n1;
     std::vector<RaggedProb> probs;
     std::vector<RaggedVirt> virt;
-    std::vector<int64_t> moff, cell_blk, mode_blk;
-    std::vector<int> vunit_off;
+    std::vector<int64_t> moff, mode_blk;
+    std::vector<int> vunit_off, unit_virt;
+    int64_t max_pieces = 0;  // pieces of every virtual problem at most
     RaggedLaunch rl{};
 
     // device tables
     void *d_probs = nullptr, *d_virt = nullptr, *d_moff = nullptr, *d_vunit_off = nullptr,
-         *d_cell_blk = nullptr, *d_mode_blk = nullptr;
+         *d_unit_virt = nullptr, *d_mode_blk = nullptr;
     void *d_ptrs = nullptr;
     const void **h_ptrs = nullptr;  // pinned staging of the pointer table
     hipEvent_t ptrs_copied = nullptr;
@@ -54,6 +55,7 @@ struct jfa_ragged_s {
     void *d_sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     int *d_bin_start = nullptr, *d_piece_start = nullptr, *d_npieces = nullptr,
+        *d_piece_unit = nullptr,
         *d_blocksum = nullptr;
 
     // fine grids and strip scratch (complex cells)
@@ -102,11 +104,12 @@ int upload_real(const std::vector<double> &h, void **d) {
 }
 
 void release(jfa_ragged r) {
-    for (void **q : {&r->d_probs, &r->d_virt, &r->d_moff, &r->d_vunit_off, &r->d_cell_blk,
+    for (void **q : {&r->d_probs, &r->d_virt, &r->d_moff, &r->d_vunit_off, &r->d_unit_virt,
                      &r->d_mode_blk, &r->d_ptrs, &r->d_coef, &r->d_dk, (void **)&r->d_idx,
                      (void **)&r->d_l0, &r->d_t, (void **)&r->d_key_in, (void **)&r->d_val_in,
                      (void **)&r->d_key_out, &r->d_sort_temp, (void **)&r->d_bin_start,
-                     (void **)&r->d_piece_start, (void **)&r->d_npieces, &r->d_fw, &r->d_scratch,
+                     (void **)&r->d_piece_start, (void **)&r->d_npieces,
+                     (void **)&r->d_piece_unit, &r->d_fw, &r->d_scratch,
                      &r->d_fft_work, (void **)&r->d_blocksum})
         free_dev(*q);
     if (r->h_ptrs) (void)hipHostFree((void *)r->h_ptrs);
@@ -200,7 +203,8 @@ int layout(jfa_ragged r) {
     }
     r->virt.clear();
     r->vunit_off.assign(1, 0);
-    r->cell_blk.assign(1, 0);
+    r->unit_virt.clear();
+    r->max_pieces = 0;
     r->mode_blk.assign(1, 0);
     r->fft.clear();
     int64_t fw = 0, tiles = 0;
@@ -226,8 +230,9 @@ int layout(jfa_ragged r) {
             r->virt.push_back(v);
             fw += F;
             tiles += strip_max_pieces(sg[i], r->npts[i]) * strip_tile_cells(sg[i]);
+            r->max_pieces += strip_max_pieces(sg[i], r->npts[i]);
+            r->unit_virt.insert(r->unit_virt.end(), sg[i].nunits, (int)r->virt.size() - 1);
             r->vunit_off.push_back(r->vunit_off.back() + sg[i].nunits);
-            r->cell_blk.push_back(r->cell_blk.back() + (F + 255) / 256);
             r->mode_blk.push_back(r->mode_blk.back() + (v.n0 * v.n1 + 255) / 256);
             r->fft.back().count++;
         }
@@ -248,7 +253,6 @@ int layout(jfa_ragged r) {
     rl.mtot = r->mtot;
     rl.nbins = bins;
     rl.nunits = r->vunit_off.back();
-    rl.cell_blocks = r->cell_blk.back();
     rl.mode_blocks = r->mode_blk.back();
     return JFA_OK;
 }
@@ -260,7 +264,7 @@ int allocate(jfa_ragged r) {
     if (int rc = upload_vec(r->virt, &r->d_virt)) return rc;
     if (int rc = upload_vec(r->moff, &r->d_moff)) return rc;
     if (int rc = upload_vec(r->vunit_off, &r->d_vunit_off)) return rc;
-    if (int rc = upload_vec(r->cell_blk, &r->d_cell_blk)) return rc;
+    if (int rc = upload_vec(r->unit_virt, &r->d_unit_virt)) return rc;
     if (int rc = upload_vec(r->mode_blk, &r->d_mode_blk)) return rc;
     JFA_HIP_CHECK(hipMalloc(&r->d_ptrs, sizeof(void *) * 4 * np));
     JFA_HIP_CHECK(hipHostMalloc((void **)&r->h_ptrs, sizeof(void *) * 4 * np, 0));
@@ -302,6 +306,7 @@ int allocate(jfa_ragged r) {
     JFA_HIP_CHECK(hipMalloc(&r->d_bin_start, sizeof(int) * (r->rl.nbins + 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_piece_start, sizeof(int) * (r->rl.nunits + 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_npieces, sizeof(int) * 4));
+    JFA_HIP_CHECK(hipMalloc(&r->d_piece_unit, sizeof(int) * std::max<int64_t>(r->max_pieces, 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_blocksum, sizeof(int) * (2 * ((r->rl.nunits + 255) / 256) + 1)));
     if (hipMalloc(&r->d_fw, 2 * real * std::max<int64_t>(r->fw_cells, 1)) != hipSuccess ||
         hipMalloc(&r->d_scratch, 2 * real * std::max<int64_t>(r->tile_cells, 1)) != hipSuccess) {
@@ -348,12 +353,13 @@ int allocate(jfa_ragged r) {
     rl.virt = (const RaggedVirt *)r->d_virt;
     rl.moff = (const int64_t *)r->d_moff;
     rl.vunit_off = (const int *)r->d_vunit_off;
-    rl.cell_blk = (const int64_t *)r->d_cell_blk;
+    rl.unit_virt = (const int *)r->d_unit_virt;
     rl.mode_blk = (const int64_t *)r->d_mode_blk;
     rl.ptrs = (const void *const *)r->d_ptrs;
     rl.bin_start = r->d_bin_start;
     rl.piece_start = r->d_piece_start;
     rl.npieces = r->d_npieces;
+    rl.piece_unit = r->d_piece_unit;
     rl.blocksum = r->d_blocksum;
     return JFA_OK;
 }

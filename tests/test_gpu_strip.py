@@ -4,8 +4,8 @@ The strip method is chosen automatically for 2D spreading; here it is forced
 (gpu_method=3) through the plan interface and checked against the FP64 direct
 sum at every kernel width class: ns <= 8 runs the modular-ownership kernel,
 wider kernels the column-per-lane kernel.  A small gpu_maxsubprobsize splits
-crowded units into pieces, which take the global-atomic flush path instead of
-the scratch tiles and merge, so both output paths are covered.
+crowded units into many pieces, each with its own scratch tile that the merge
+sums, so the multi-piece path is covered; the output is bitwise reproducible.
 """
 
 from itertools import product
@@ -91,3 +91,17 @@ def test_auto_method_is_strip_for_2d_type1():
     a, _ = _type1_plan((100, 120), x, c, 1e-6, "f32", gpu_method=3)
     b = host(nufft1((100, 120), dev(c), *[dev(v) for v in x], eps=1e-6, opts=Opts()))
     assert rel_l2(a[0], b) < 1e-7
+
+
+@pytest.mark.parametrize("pmax", [0, 50])
+def test_strip_is_bitwise_reproducible(pmax):
+    # no float atomics: the same inputs give the same bits, crowded units included
+    rng = np.random.default_rng(25)
+    M = 30000
+    x = clustered_points(rng, 2, M, "f32")
+    c = cnormal(rng, (2, M), "f32")
+    a, _ = _type1_plan((128, 96), x, c, 1e-6, "f32", ntransf=2, gpu_method=3,
+                       gpu_maxsubprobsize=pmax)
+    b, _ = _type1_plan((128, 96), x, c, 1e-6, "f32", ntransf=2, gpu_method=3,
+                       gpu_maxsubprobsize=pmax)
+    assert np.array_equal(a.view(np.float32), b.view(np.float32))
