@@ -1,0 +1,362 @@
+// Pruned multi-dimensional FFT between the fine grid and the modes, for 2D and
+// 3D plans.  Replaces the full nf0 x nf1 (x nf2) FFT cuFINUFFT runs inside
+// execute (called at lib/kernels.cc.cu:75; source absent from the reference)
+// with 1D passes that skip the columns no mode needs.
+//
+// Of the nf_d fine-grid frequencies along an axis only the n_d mode
+// frequencies k in [-n_d/2, (n_d-1)/2] are kept; they sit at fine index
+// k mod nf_d.  Compact index j in [0, n_d) stands for k = j (j < (n_d+1)/2)
+// or k = j - n_d, i.e. fine index sel(j) = j or nf_d - n_d + j.
+//
+// Type 1 (fine grid -> modes), 2D: x-FFT of every row (length nf0); gather the
+// n0 needed columns transposed into G[t][j0][y]; y-FFT of those n0 rows
+// (length nf1); deconvolve from G.  3D: x-FFT; gather into
+// G1[t][z][j0][y]; y-FFT; gather the n1 needed y of each (z, j0) transposed
+// into G2[t][j0][j1][z] (in the fine grid's memory, dead by then); z-FFT;
+// deconvolve from G2.  Type 2 runs the same steps backwards: amplify into the
+// compact buffer (zero outside the mode rows), FFT, scatter-transpose into the
+// next larger buffer writing zeros in the unused columns, ..., x-FFT of the
+// full grid.  Data moved per transform, 2D: 2F + F/2 + F/2 + F/2 against 8F
+// for rocFFT's row-FFT / transpose / row-FFT / transpose of the full grid.
+#include "jfa_device.hpp"
+#include "plan_state.hpp"
+
+namespace jfa {
+namespace {
+
+__device__ __forceinline__ int64_t freq_of(int64_t i, int64_t n, int modeord) {
+    if (modeord == 0) return i - n / 2;
+    return i < (n + 1) / 2 ? i : i - n;
+}
+
+// compact index of frequency k (|k| within the modes) and back
+__device__ __forceinline__ int64_t compact_of(int64_t k, int64_t n) { return k < 0 ? k + n : k; }
+__device__ __forceinline__ int64_t sel_of(int64_t j, int64_t n, int64_t nf) {
+    return j < (n + 1) / 2 ? j : nf - n + j;
+}
+// compact index of fine index c, or -1 when no mode lives there
+__device__ __forceinline__ int64_t unsel_of(int64_t c, int64_t n, int64_t nf) {
+    if (c < (n + 1) / 2) return c;
+    if (c >= nf - n / 2) return c - nf + n;
+    return -1;
+}
+
+constexpr int kTile = 32;
+
+// B[o][j][r] = A[o][r][col(j)] for j < J, r < R; col(j) = (j / nsel) * cblk +
+// sel_of(j % nsel, nsel, cblk); A rows hold C reals pairs.  One 32 x 32 tile of
+// (j, r) per workgroup of 32 x 8 threads, through LDS.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gather_cols_kernel(const T *__restrict__ A, T *__restrict__ B, int64_t R, int64_t C,
+                       int64_t J, int64_t nsel, int64_t cblk, int64_t jt, int64_t rt) {
+    __shared__ T tr[kTile][kTile + 1], ti[kTile][kTile + 1];
+    int64_t tile = blockIdx.x;
+    const int64_t jb = tile % jt;
+    tile /= jt;
+    const int64_t rb = tile % rt;
+    const int64_t o = tile / rt;
+    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+    A += 2 * o * R * C;
+    B += 2 * o * J * R;
+    // read: lanes along j (columns of A), rows r
+    const int64_t j = jb * kTile + tx;
+    int64_t col = -1;
+    if (j < J) col = (j / nsel) * cblk + sel_of(j % nsel, nsel, cblk);
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t r = rb * kTile + k;
+        T vr = T(0), vi = T(0);
+        if (col >= 0 && r < R) {
+            vr = A[2 * (r * C + col)];
+            vi = A[2 * (r * C + col) + 1];
+        }
+        tr[k][tx] = vr;
+        ti[k][tx] = vi;
+    }
+    __syncthreads();
+    // write: lanes along r (rows of B)
+    const int64_t r = rb * kTile + tx;
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t jj = jb * kTile + k;
+        if (jj < J && r < R) {
+            B[2 * (jj * R + r)] = tr[tx][k];
+            B[2 * (jj * R + r) + 1] = ti[tx][k];
+        }
+    }
+}
+
+// A[o][r][c] = B[o][unsel(c)][r] for every c < C (0 where no compact column
+// maps to c); the inverse of gather_cols_kernel, writing every element of A.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    scatter_cols_kernel(const T *__restrict__ B, T *__restrict__ A, int64_t R, int64_t C,
+                        int64_t J, int64_t nsel, int64_t cblk, int64_t ct, int64_t rt) {
+    __shared__ T tr[kTile][kTile + 1], ti[kTile][kTile + 1];
+    int64_t tile = blockIdx.x;
+    const int64_t cb = tile % ct;
+    tile /= ct;
+    const int64_t rb = tile % rt;
+    const int64_t o = tile / rt;
+    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+    A += 2 * o * R * C;
+    B += 2 * o * J * R;
+    // read: lanes along r (rows of B), one compact column per k
+    const int64_t r = rb * kTile + tx;
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t c = cb * kTile + k;
+        T vr = T(0), vi = T(0);
+        if (c < C && r < R) {
+            const int64_t u = unsel_of(c % cblk, nsel, cblk);
+            if (u >= 0) {
+                const int64_t jj = (c / cblk) * nsel + u;
+                vr = B[2 * (jj * R + r)];
+                vi = B[2 * (jj * R + r) + 1];
+            }
+        }
+        tr[k][tx] = vr;
+        ti[k][tx] = vi;
+    }
+    __syncthreads();
+    // write: lanes along c (rows of A)
+    const int64_t c = cb * kTile + tx;
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t rr = rb * kTile + k;
+        if (c < C && rr < R) {
+            A[2 * (rr * C + c)] = tr[tx][k];
+            A[2 * (rr * C + c) + 1] = ti[tx][k];
+        }
+    }
+}
+
+// out[t][i] = G[t][...] * prod_d dk_d[k_d + n_d/2]; G is [n0][nf1] (2D) or
+// [n0][n1][nf2] (3D) per transform, the last axis at full length.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+    deconv_pruned_kernel(int64_t n0, int64_t n1, int64_t n2, int64_t nf1, int64_t nf2,
+                         const T *__restrict__ dk0, const T *__restrict__ dk1,
+                         const T *__restrict__ dk2, int modeord, const T *__restrict__ G,
+                         int64_t g_stride, T *__restrict__ out, int64_t out_stride) {
+    const int64_t ntot = n0 * n1 * n2;
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= ntot) return;
+    G += 2 * g_stride * blockIdx.y;
+    out += 2 * out_stride * blockIdx.y;
+    const int64_t k0 = freq_of(q % n0, n0, modeord);
+    const int64_t k1 = freq_of((q / n0) % n1, n1, modeord);
+    T s = dk0[k0 + n0 / 2] * dk1[k1 + n1 / 2];
+    int64_t l;
+    if constexpr (DIM == 2) {
+        l = compact_of(k0, n0) * nf1 + (k1 < 0 ? k1 + nf1 : k1);
+    } else {
+        const int64_t k2 = freq_of(q / (n0 * n1), n2, modeord);
+        s *= dk2[k2 + n2 / 2];
+        l = (compact_of(k0, n0) * n1 + compact_of(k1, n1)) * nf2 + (k2 < 0 ? k2 + nf2 : k2);
+    }
+    out[2 * q] = G[2 * l] * s;
+    out[2 * q + 1] = G[2 * l + 1] * s;
+}
+
+// G (layout as above) from the modes: mode value times 1/psi_hat where the
+// full-length last axis holds a mode frequency, 0 elsewhere; every element of
+// G is written.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+    amplify_pruned_kernel(int64_t n0, int64_t n1, int64_t n2, int64_t nf1, int64_t nf2,
+                          const T *__restrict__ dk0, const T *__restrict__ dk1,
+                          const T *__restrict__ dk2, int modeord, const T *__restrict__ f,
+                          int64_t f_stride, T *__restrict__ G, int64_t g_stride) {
+    const int64_t last = DIM == 2 ? nf1 : nf2;
+    const int64_t gsize = DIM == 2 ? n0 * nf1 : n0 * n1 * nf2;
+    const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (l >= gsize) return;
+    f += 2 * f_stride * blockIdx.y;
+    G += 2 * g_stride * blockIdx.y;
+    const int64_t c = l % last;  // fine index along the last axis
+    const int64_t nl = DIM == 2 ? n1 : n2;
+    const int64_t kl = c < (last + 1) / 2 ? c : c - last;
+    T vr = T(0), vi = T(0);
+    if (kl >= -(nl / 2) && kl <= (nl - 1) / 2) {
+        const int64_t rest = l / last;
+        int64_t j0, j1 = 0;
+        if constexpr (DIM == 2) {
+            j0 = rest;
+        } else {
+            j0 = rest / n1;
+            j1 = rest % n1;
+        }
+        const int64_t k0 = j0 < (n0 + 1) / 2 ? j0 : j0 - n0;
+        auto idx_of = [&](int64_t k, int64_t n) { return modeord == 0 ? k + n / 2 : (k < 0 ? k + n : k); };
+        T s = dk0[k0 + n0 / 2];
+        int64_t idx = idx_of(k0, n0);
+        if constexpr (DIM == 2) {
+            s *= dk1[kl + n1 / 2];
+            idx += n0 * idx_of(kl, n1);
+        } else {
+            const int64_t k1 = j1 < (n1 + 1) / 2 ? j1 : j1 - n1;
+            s *= dk1[k1 + n1 / 2] * dk2[kl + n2 / 2];
+            idx += n0 * (idx_of(k1, n1) + n1 * idx_of(kl, n2));
+        }
+        vr = f[2 * idx] * s;
+        vi = f[2 * idx + 1] * s;
+    }
+    G[2 * l] = vr;
+    G[2 * l + 1] = vi;
+}
+
+template <typename T>
+int gather_cols(const T *A, T *B, int64_t outer, int64_t R, int64_t C, int64_t J, int64_t nsel,
+                int64_t cblk, hipStream_t s) {
+    const int64_t jt = (J + kTile - 1) / kTile, rt = (R + kTile - 1) / kTile;
+    const int64_t blocks = jt * rt * outer;
+    if (blocks == 0) return JFA_OK;
+    if (blocks > INT32_MAX) {
+        set_error("pruned FFT transpose of %lld tiles is too large", (long long)blocks);
+        return JFA_ERR_TOO_LARGE;
+    }
+    gather_cols_kernel<T><<<(unsigned)blocks, 256, 0, s>>>(A, B, R, C, J, nsel, cblk, jt, rt);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
+int scatter_cols(const T *B, T *A, int64_t outer, int64_t R, int64_t C, int64_t J, int64_t nsel,
+                 int64_t cblk, hipStream_t s) {
+    const int64_t ct = (C + kTile - 1) / kTile, rt = (R + kTile - 1) / kTile;
+    const int64_t blocks = ct * rt * outer;
+    if (blocks == 0) return JFA_OK;
+    if (blocks > INT32_MAX) {
+        set_error("pruned FFT transpose of %lld tiles is too large", (long long)blocks);
+        return JFA_ERR_TOO_LARGE;
+    }
+    scatter_cols_kernel<T><<<(unsigned)blocks, 256, 0, s>>>(B, A, R, C, J, nsel, cblk, ct, rt);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+// complex elements per transform of the separate compact buffer: G (2D) or
+// G1 (3D)
+int64_t fwt_cells(jfa_plan p) {
+    const int64_t *n = p->nmodes, *nf = p->geom.nf;
+    return p->dim == 2 ? n[0] * nf[1] : n[0] * nf[1] * nf[2];
+}
+
+}  // namespace
+
+bool fft_is_pruned(jfa_plan p) { return p->dim >= 2; }
+
+int reserve_fwt(jfa_plan p) {
+    const size_t bytes = (size_t)p->batch * fwt_cells(p) * 2 * (p->is_double ? 8 : 4);
+    if (bytes <= p->fwt_bytes) return JFA_OK;
+    if (p->d_fwt) (void)hipFree(p->d_fwt);
+    p->d_fwt = nullptr;
+    p->fwt_bytes = 0;
+    if (hipMalloc(&p->d_fwt, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("could not allocate %zu bytes for the pruned FFT", bytes);
+        return JFA_ERR_ALLOC;
+    }
+    p->fwt_bytes = bytes;
+    return JFA_OK;
+}
+
+// The compact mode buffer the deconvolution reads / the amplification writes.
+template <typename T>
+static T *compact_modes(jfa_plan p, T *fw) {
+    return p->dim == 2 ? (T *)p->d_fwt : fw;
+}
+
+template <typename T>
+int pruned_grid_fft(jfa_plan p, T *fw, int nb) {
+    if (int rc = reserve_fwt(p)) return rc;
+    const int64_t *n = p->nmodes, *nf = p->geom.nf;
+    T *G = (T *)p->d_fwt;
+    if (int rc = fft1d(p, fw, nf[0], (int64_t)nb * nf[1] * nf[2])) return rc;
+    if (p->dim == 2) {
+        if (int rc = gather_cols<T>(fw, G, nb, nf[1], nf[0], n[0], n[0], nf[0], p->stream))
+            return rc;
+        return fft1d(p, G, nf[1], (int64_t)nb * n[0]);
+    }
+    // 3D: G1[t][z][j0][y], then G2[t][j0][j1][z] in fw
+    if (int rc = gather_cols<T>(fw, G, (int64_t)nb * nf[2], nf[1], nf[0], n[0], n[0], nf[0],
+                                p->stream))
+        return rc;
+    if (int rc = fft1d(p, G, nf[1], (int64_t)nb * nf[2] * n[0])) return rc;
+    if (int rc = gather_cols<T>(G, fw, nb, nf[2], n[0] * nf[1], n[0] * n[1], n[1], nf[1],
+                                p->stream))
+        return rc;
+    return fft1d(p, fw, nf[2], (int64_t)nb * n[0] * n[1]);
+}
+
+template <typename T>
+int pruned_modes_fft(jfa_plan p, T *fw, int nb) {
+    if (int rc = reserve_fwt(p)) return rc;
+    const int64_t *n = p->nmodes, *nf = p->geom.nf;
+    T *G = (T *)p->d_fwt;
+    if (p->dim == 2) {
+        if (int rc = fft1d(p, G, nf[1], (int64_t)nb * n[0])) return rc;
+        if (int rc = scatter_cols<T>(G, fw, nb, nf[1], nf[0], n[0], n[0], nf[0], p->stream))
+            return rc;
+        return fft1d(p, fw, nf[0], (int64_t)nb * nf[1]);
+    }
+    // 3D: G2 (in fw) -> G1 -> fw
+    if (int rc = fft1d(p, fw, nf[2], (int64_t)nb * n[0] * n[1])) return rc;
+    if (int rc = scatter_cols<T>(fw, G, nb, nf[2], n[0] * nf[1], n[0] * n[1], n[1], nf[1],
+                                 p->stream))
+        return rc;
+    if (int rc = fft1d(p, G, nf[1], (int64_t)nb * nf[2] * n[0])) return rc;
+    if (int rc = scatter_cols<T>(G, fw, (int64_t)nb * nf[2], nf[1], nf[0], n[0], n[0], nf[0],
+                                 p->stream))
+        return rc;
+    return fft1d(p, fw, nf[0], (int64_t)nb * nf[1] * nf[2]);
+}
+
+template <typename T>
+int pruned_deconv(jfa_plan p, T *fw, T *fk, int nb) {
+    const int64_t *n = p->nmodes, *nf = p->geom.nf;
+    const int64_t ntot = n[0] * n[1] * n[2];
+    if (ntot == 0 || nb == 0) return JFA_OK;
+    const T *G = compact_modes<T>(p, fw);
+    const int64_t gs = p->dim == 2 ? fwt_cells(p) : n[0] * n[1] * nf[2];
+    const T *dk[3] = {(const T *)p->d_dk[0], (const T *)p->d_dk[1], (const T *)p->d_dk[2]};
+    dim3 grid((unsigned)((ntot + 255) / 256), nb);
+    if (p->dim == 2)
+        deconv_pruned_kernel<T, 2><<<grid, 256, 0, p->stream>>>(
+            n[0], n[1], 1, nf[1], 1, dk[0], dk[1], dk[2], p->opts.modeord, G, gs, fk, p->Ntot);
+    else
+        deconv_pruned_kernel<T, 3><<<grid, 256, 0, p->stream>>>(
+            n[0], n[1], n[2], nf[1], nf[2], dk[0], dk[1], dk[2], p->opts.modeord, G, gs, fk,
+            p->Ntot);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
+int pruned_amplify(jfa_plan p, const T *fk, T *fw, int nb) {
+    if (int rc = reserve_fwt(p)) return rc;
+    const int64_t *n = p->nmodes, *nf = p->geom.nf;
+    T *G = compact_modes<T>(p, fw);
+    const int64_t gs = p->dim == 2 ? fwt_cells(p) : n[0] * n[1] * nf[2];
+    if (gs == 0 || nb == 0) return JFA_OK;
+    const T *dk[3] = {(const T *)p->d_dk[0], (const T *)p->d_dk[1], (const T *)p->d_dk[2]};
+    dim3 grid((unsigned)((gs + 255) / 256), nb);
+    if (p->dim == 2)
+        amplify_pruned_kernel<T, 2><<<grid, 256, 0, p->stream>>>(
+            n[0], n[1], 1, nf[1], 1, dk[0], dk[1], dk[2], p->opts.modeord, fk, p->Ntot, G, gs);
+    else
+        amplify_pruned_kernel<T, 3><<<grid, 256, 0, p->stream>>>(
+            n[0], n[1], n[2], nf[1], nf[2], dk[0], dk[1], dk[2], p->opts.modeord, fk, p->Ntot,
+            G, gs);
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+#define JFA_INST(T)                                                 \
+    template int pruned_grid_fft<T>(jfa_plan, T *, int);            \
+    template int pruned_modes_fft<T>(jfa_plan, T *, int);           \
+    template int pruned_deconv<T>(jfa_plan, T *, T *, int);         \
+    template int pruned_amplify<T>(jfa_plan, const T *, T *, int);
+JFA_INST(float)
+JFA_INST(double)
+#undef JFA_INST
+
+}  // namespace jfa

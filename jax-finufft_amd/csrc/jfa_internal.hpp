@@ -116,6 +116,9 @@ struct RaggedProb {
 struct RaggedVirt {
     StripGeom sg;
     int prob, t;
+    int nt;            // transforms this one spreads: 2 (itself and t + 1, which
+                       // shares its points), 1, or 0 (spread with t - 1)
+    int64_t tile_next; // tile_off of transform t + 1 minus this tile_off
     int64_t m;         // the problem's point count
     int64_t bin_off;   // the problem's first global bin
     int64_t fw_off;    // into the concatenated fine grids
@@ -176,6 +179,7 @@ struct StripPts {
     const T *rec;         // (tx, ty) of input point P at rec[2*P], written by setpts
     int64_t bin_off;      // the problem's first global bin (0 for one plan)
     int64_t idx_off;      // the problem's first input point: j = P - idx_off
+    int64_t c_tstride;    // reals from one transform's strengths to the next (c path)
 };
 
 // Per-bin bookkeeping for the LDS-subproblem method.

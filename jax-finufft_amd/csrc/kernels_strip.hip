@@ -53,17 +53,18 @@ struct alignas(4 * sizeof(T)) Quad {
     T x, y, z, w;
 };
 
-// LDS staging of one wave: ns <= 8 uses the weight pairs, wider kernels the
-// column weights and c * y-weights per point.
-template <typename T, int NS, bool MOD8 = (NS <= 8)>
+// LDS staging of one wave: ns <= 8 uses the weight pairs (and c * y-weights
+// for NT transforms at once), wider kernels the column weights and c *
+// y-weights per point.
+template <typename T, int NS, int NT = 1, bool MOD8 = (NS <= 8)>
 struct StripLds;
-template <typename T, int NS>
-struct StripLds<T, NS, true> {
-    Pair<T> w[kStripLanes * 8];    // per point q, lane column a: (weight in bx, in bx+1)
-    Pair<T> cky[kStripLanes * 8];  // per point q, row offset dy: c * ky[dy] (0 if dy >= NS)
+template <typename T, int NS, int NT>
+struct StripLds<T, NS, NT, true> {
+    Pair<T> w[kStripLanes * 8];        // per point q, lane column a: (weight in bx, in bx+1)
+    Pair<T> cky[NT][kStripLanes * 8];  // per transform, point q, row offset dy: c * ky[dy]
 };
-template <typename T, int NS>
-struct StripLds<T, NS, false> {
+template <typename T, int NS, int NT>
+struct StripLds<T, NS, NT, false> {
     T kx[kStripLanes * NS];
     T cky[kStripLanes * NS * 2];
     int lx[kStripLanes];
@@ -102,11 +103,11 @@ __device__ __forceinline__ PieceRange piece_range(const StripGeom &sg,
 
 // ------------------------------------------------------------ ns > 8 body
 
-template <typename T, int NS>
+template <typename T, int NS, typename Lds>
 __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T> &pts,
                                            const int *__restrict__ bin_start, int u, int sub,
                                            const T *__restrict__ coef, T beta, int horner,
-                                           T *__restrict__ tile, StripLds<T, NS> &lds) {
+                                           T *__restrict__ tile, Lds &lds) {
     const int lane = threadIdx.x;
     const int rows_tile = sg.H + NS - 1;
     const PieceRange pr = piece_range(sg, bin_start, u, sub);
@@ -201,31 +202,40 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
 // switch; the second by staging, per point and lane column, the weight split
 // into a pair (weight in block bx, weight in block bx + 1), one of them zero.
 
-template <typename T, int BX>
+template <typename T, int BX, int NT>
 __device__ __forceinline__ void strip8_walk(const Pair<T> *__restrict__ s_w,
-                                            const Pair<T> *__restrict__ s_cky, int q0, int q1,
-                                            int a, int dy, T (&acc_r)[8], T (&acc_i)[8]) {
+                                            const Pair<T> (*__restrict__ s_cky)[kStripLanes * 8],
+                                            int q0, int q1, int a, int dy, T (&acc_r)[NT][8],
+                                            T (&acc_i)[NT][8]) {
 #pragma unroll 4
     for (int q = q0; q < q1; ++q) {
         const Pair<T> w = s_w[q * 8 + a];
-        const Pair<T> cy = s_cky[q * 8 + dy];
-        acc_r[BX] = fma(w.x, cy.x, acc_r[BX]);
-        acc_i[BX] = fma(w.x, cy.y, acc_i[BX]);
-        if constexpr (BX < 7) {  // block 8 would only receive zero weights
-            acc_r[BX + 1] = fma(w.y, cy.x, acc_r[BX + 1]);
-            acc_i[BX + 1] = fma(w.y, cy.y, acc_i[BX + 1]);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const Pair<T> cy = s_cky[t][q * 8 + dy];
+            acc_r[t][BX] = fma(w.x, cy.x, acc_r[t][BX]);
+            acc_i[t][BX] = fma(w.x, cy.y, acc_i[t][BX]);
+            if constexpr (BX < 7) {  // block 8 would only receive zero weights
+                acc_r[t][BX + 1] = fma(w.y, cy.x, acc_r[t][BX + 1]);
+                acc_i[t][BX + 1] = fma(w.y, cy.y, acc_i[t][BX + 1]);
+            }
         }
     }
 }
 
-template <typename T, int NS>
+// One piece for NT transforms sharing the points: transform k's strengths at
+// pts.c + k * pts.c_tstride (NT = 1 may read packed records instead), its tile
+// at tile + k * tile_tstride.
+template <typename T, int NS, int NT, int LNT>
 __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts<T> &pts,
                                              const int *__restrict__ bin_start, int u, int sub,
                                              const T *__restrict__ coef, T beta, int horner,
-                                             T *__restrict__ tile, StripLds<T, NS> &lds) {
+                                             T *__restrict__ tile, int64_t tile_tstride,
+                                             StripLds<T, NS, LNT> &lds) {
     static_assert(NS <= 8, "modular ownership needs ns <= 8");
+    static_assert(NT <= LNT, "LDS staging for NT transforms");
     Pair<T> *s_w = lds.w;
-    Pair<T> *s_cky = lds.cky;
+    Pair<T> (*s_cky)[kStripLanes * 8] = lds.cky;
     const int lane = threadIdx.x;
     const int a = lane & 7, r = lane >> 3;
     const int rows_tile = sg.H + NS - 1;
@@ -233,21 +243,24 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
     const int r0 = pr.r0, pb = pr.pb, pe = pr.pe;
     // key >> kStripKeyShift = rowbase + l0y for every point of this strip
     const int64_t rowbase = pts.bin_off + (int64_t)(u / sg.K) * sg.nf1;
-    T acc_r[8], acc_i[8];
+    T acc_r[NT][8], acc_i[NT][8];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc_r[t][b] = acc_i[t][b] = T(0);
     // the window holds rows R .. R+7; row y sits in the lanes with r == y & 7
     int R = r0;
     auto flush = [&](int row) {
         if ((row & 7) == r) {
-            {
-                Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile) +
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile + t * tile_tstride) +
                                (int64_t)(row - r0) * kStripLanes + a;
 #pragma unroll
-                for (int b = 0; b < 8; ++b) dst[8 * b] = Pair<T>{acc_r[b], acc_i[b]};
-            }
+                for (int b = 0; b < 8; ++b) dst[8 * b] = Pair<T>{acc_r[t][b], acc_i[t][b]};
 #pragma unroll
-            for (int b = 0; b < 8; ++b) acc_r[b] = acc_i[b] = T(0);
+                for (int b = 0; b < 8; ++b) acc_r[t][b] = acc_i[t][b] = T(0);
+            }
         }
     };
     for (int bb = pb; bb < pe; bb += kStripLanes) {
@@ -283,8 +296,17 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
                     eval_kernel<T, NS, false>(coef, beta, tt.y, ky);
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
-                    s_cky[lane * 8 + i] =
+                    s_cky[0][lane * 8 + i] =
                         i < NS ? Pair<T>{cr * ky[i], ci * ky[i]} : Pair<T>{T(0), T(0)};
+#pragma unroll
+                for (int t = 1; t < NT; ++t) {
+                    const Pair<T> ct = reinterpret_cast<const Pair<T> *>(
+                        pts.c + t * pts.c_tstride)[pts.idx[p] - pts.idx_off];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i)
+                        s_cky[t][lane * 8 + i] = i < NS ? Pair<T>{ct.x * ky[i], ct.y * ky[i]}
+                                                        : Pair<T>{T(0), T(0)};
+                }
             }
             key = ((int)((pkey >> kStripKeyShift) - rowbase) << 3) | (X >> 3);
         }
@@ -304,14 +326,14 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
             }
             const int dy = (r - Y) & 7;
             switch (gkey & 7) {
-                case 0: strip8_walk<T, 0>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 1: strip8_walk<T, 1>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 2: strip8_walk<T, 2>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 3: strip8_walk<T, 3>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 4: strip8_walk<T, 4>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 5: strip8_walk<T, 5>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 6: strip8_walk<T, 6>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                default: strip8_walk<T, 7>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 0: strip8_walk<T, 0, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 1: strip8_walk<T, 1, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 2: strip8_walk<T, 2, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 3: strip8_walk<T, 3, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 4: strip8_walk<T, 4, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 5: strip8_walk<T, 5, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 6: strip8_walk<T, 6, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                default: strip8_walk<T, 7, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
             }
         }
         __syncthreads();
@@ -324,15 +346,28 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
     }
 }
 
-template <typename T, int NS>
+// One piece for nt transforms (nt <= LNT) sharing the points (see strip8_piece).
+template <typename T, int NS, int LNT>
 __device__ __forceinline__ void strip_piece(const StripGeom &sg, const StripPts<T> &pts,
                                             const int *__restrict__ bin_start, int u, int sub,
                                             const T *__restrict__ coef, T beta, int horner,
-                                            T *__restrict__ tile, StripLds<T, NS> &lds) {
-    if constexpr (NS <= 8)
-        strip8_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner, tile, lds);
-    else
-        wide_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner, tile, lds);
+                                            T *__restrict__ tile, int nt, int64_t tile_tstride,
+                                            StripLds<T, NS, LNT> &lds) {
+    if constexpr (NS <= 8) {
+        if (LNT > 1 && nt > 1)
+            strip8_piece<T, NS, LNT, LNT>(sg, pts, bin_start, u, sub, coef, beta, horner, tile,
+                                          tile_tstride, lds);
+        else
+            strip8_piece<T, NS, 1, LNT>(sg, pts, bin_start, u, sub, coef, beta, horner, tile,
+                                        tile_tstride, lds);
+    } else {
+        StripPts<T> pk = pts;
+        for (int k = 0; k < nt; ++k) {
+            wide_piece<T, NS>(sg, pk, bin_start, u, sub, coef, beta, horner,
+                              tile + k * tile_tstride, lds);
+            pk.c += pts.c_tstride;
+        }
+    }
 }
 
 // The windows of one axis that reach into window w's first ns - 1 cells from
@@ -472,7 +507,7 @@ __global__ void __launch_bounds__(kStripLanes)
         const int u = piece_unit[piece];
         const int sub = piece - piece_start[u];
         strip_piece<T, NS>(sg, pts, bin_start, u, sub, coef, beta, horner,
-                           scratch + 2 * piece * tile_cells, lds);
+                           scratch + 2 * piece * tile_cells, 1, 0, lds);
     }
 }
 
@@ -591,21 +626,23 @@ __global__ void __launch_bounds__(kStripLanes)
                         const T *__restrict__ coef, T beta, int horner,
                         const void *const *__restrict__ ptrs, int nprob,
                         T *__restrict__ scratch_all) {
-    __shared__ StripLds<T, NS> lds;
+    __shared__ StripLds<T, NS, 2> lds;
     const int total = *npieces;
     for (int piece = blockIdx.x; piece < total; piece += gridDim.x) {
         const int gu = piece_unit[piece];
         const int v = unit_virt[gu];
         const RaggedVirt d = virt[v];
+        if (d.nt == 0) continue;  // spread together with transform t - 1
         const int sub = piece - piece_start[gu];
         StripPts<T> vp = pts;
         vp.c = (const T *)ptrs[2 * nprob + d.prob] + 2 * (int64_t)d.t * d.m;
+        vp.c_tstride = 2 * d.m;
         vp.bin_off = d.bin_off;
         vp.idx_off = d.moff;
         const int64_t tile_cells = (int64_t)(d.sg.H + NS - 1) * kStripLanes;
         T *tile = scratch_all + 2 * (d.tile_off + (piece - piece_start[d.unit_off]) * tile_cells);
         strip_piece<T, NS>(d.sg, vp, bin_start + d.bin_off, gu - d.unit_off, sub, coef, beta,
-                           horner, tile, lds);
+                           horner, tile, d.nt, 2 * d.tile_next, lds);
     }
 }
 

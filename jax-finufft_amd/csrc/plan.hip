@@ -230,6 +230,38 @@ static int reserve_strip_scratch(jfa_plan p, int64_t M) {
 
 // ------------------------------------------------------------------ fft
 
+// Work buffer large enough for `plan`, shared by all of the plan's FFTs.
+static int grow_fft_work(jfa_plan p, rocfft_plan plan) {
+    size_t ws = 0;
+    rocfft_plan_get_work_buffer_size(plan, &ws);
+    if (ws > p->fft_work_bytes) {
+        free_ptr(p->d_fft_work);
+        p->fft_work_bytes = 0;
+        JFA_HIP_CHECK(hipMalloc(&p->d_fft_work, ws));
+        p->fft_work_bytes = ws;
+    }
+    return JFA_OK;
+}
+
+static int run_fft(jfa_plan p, rocfft_plan plan, void *data) {
+    if (!p->fft_info) {
+        if (rocfft_execution_info_create(&p->fft_info) != rocfft_status_success) {
+            set_error("rocfft_execution_info_create failed");
+            return JFA_ERR_FFT;
+        }
+    }
+    rocfft_execution_info_set_stream(p->fft_info, p->stream);
+    if (p->fft_work_bytes)
+        rocfft_execution_info_set_work_buffer(p->fft_info, p->d_fft_work, p->fft_work_bytes);
+    void *bufs[1] = {data};
+    rocfft_status st = rocfft_execute(plan, bufs, nullptr, p->fft_info);
+    if (st != rocfft_status_success) {
+        set_error("rocfft_execute failed (status %d)", (int)st);
+        return JFA_ERR_FFT;
+    }
+    return JFA_OK;
+}
+
 int jfa::fft_exec(jfa_plan p, void *data, int nbatch) {
     if (nbatch <= 0) return JFA_OK;
     rocfft_plan plan = nullptr;
@@ -247,33 +279,44 @@ int jfa::fft_exec(jfa_plan p, void *data, int nbatch) {
             set_error("rocfft_plan_create failed (status %d)", (int)st);
             return JFA_ERR_FFT;
         }
-        size_t ws = 0;
-        rocfft_plan_get_work_buffer_size(plan, &ws);
-        if (ws > p->fft_work_bytes) {
-            free_ptr(p->d_fft_work);
-            JFA_HIP_CHECK(hipMalloc(&p->d_fft_work, ws));
-            p->fft_work_bytes = ws;
+        if (int rc = grow_fft_work(p, plan)) {
+            rocfft_plan_destroy(plan);
+            return rc;
         }
         p->fft_plans[nbatch] = plan;
     } else {
         plan = it->second;
     }
-    if (!p->fft_info) {
-        if (rocfft_execution_info_create(&p->fft_info) != rocfft_status_success) {
-            set_error("rocfft_execution_info_create failed");
+    return run_fft(p, plan, data);
+}
+
+int jfa::fft1d(jfa_plan p, void *data, int64_t len, int64_t batch) {
+    if (batch <= 0) return JFA_OK;
+    const auto key = std::make_pair(len, batch);
+    rocfft_plan plan = nullptr;
+    auto it = p->fft1_plans.find(key);
+    if (it == p->fft1_plans.end()) {
+        const size_t lengths[1] = {(size_t)len};
+        rocfft_status st = rocfft_plan_create(
+            &plan, rocfft_placement_inplace,
+            p->iflag > 0 ? rocfft_transform_type_complex_inverse
+                         : rocfft_transform_type_complex_forward,
+            p->is_double ? rocfft_precision_double : rocfft_precision_single, 1, lengths,
+            (size_t)batch, nullptr);
+        if (st != rocfft_status_success) {
+            set_error("rocfft_plan_create (1D, length %lld, batch %lld) failed (status %d)",
+                      (long long)len, (long long)batch, (int)st);
             return JFA_ERR_FFT;
         }
+        if (int rc = grow_fft_work(p, plan)) {
+            rocfft_plan_destroy(plan);
+            return rc;
+        }
+        p->fft1_plans[key] = plan;
+    } else {
+        plan = it->second;
     }
-    rocfft_execution_info_set_stream(p->fft_info, p->stream);
-    if (p->fft_work_bytes)
-        rocfft_execution_info_set_work_buffer(p->fft_info, p->d_fft_work, p->fft_work_bytes);
-    void *bufs[1] = {data};
-    rocfft_status st = rocfft_execute(plan, bufs, nullptr, p->fft_info);
-    if (st != rocfft_status_success) {
-        set_error("rocfft_execute failed (status %d)", (int)st);
-        return JFA_ERR_FFT;
-    }
-    return JFA_OK;
+    return run_fft(p, plan, data);
 }
 
 // ------------------------------------------------------------------ timing
@@ -775,6 +818,28 @@ int jfa::spread_any(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t fw_
 template int jfa::spread_any<float>(jfa_plan, const float *, int64_t, float *, int64_t, int);
 template int jfa::spread_any<double>(jfa_plan, const double *, int64_t, double *, int64_t, int);
 
+// Amplify nb transforms of modes (stride Ntot) onto the fine grid fw (stride
+// F) and FFT them: the type-2 / interpolation input.
+template <typename T>
+static int modes_to_grid(jfa_plan p, const T *fk, T *fw, int nb, StageTimer &tm) {
+    const GridGeom &g = p->geom;
+    int rc;
+    if (fft_is_pruned(p)) {
+        if ((rc = pruned_amplify<T>(p, fk, fw, nb))) return rc;
+        tm.mark("amplify");
+        if ((rc = pruned_modes_fft<T>(p, fw, nb))) return rc;
+    } else {
+        const T *dk[3] = {(const T *)p->d_dk[0], (const T *)p->d_dk[1], (const T *)p->d_dk[2]};
+        if ((rc = launch_amplify<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2],
+                                    p->opts.modeord, fk, p->Ntot, fw, p->F, nb, p->stream)))
+            return rc;
+        tm.mark("amplify");
+        if ((rc = fft_exec(p, fw, nb))) return rc;
+    }
+    tm.mark("fft");
+    return JFA_OK;
+}
+
 // Type 1 / type 2 execution on the plan's current points.  c has ntransf*M
 // complex values, fk ntransf*Ntot.
 template <typename T>
@@ -816,19 +881,21 @@ int jfa::execute_12(jfa_plan p, T *c, T *fk) {
                 tm.mark("zero");
             }
             if ((rc = spread_timed<T>(p, cb, M, fw, p->F, nb, tm.on ? &tm : nullptr))) return rc;
-            if ((rc = fft_exec(p, fw, nb))) return rc;
-            tm.mark("fft");
-            if ((rc = launch_deconv<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2],
-                                       p->opts.modeord, fw, p->F, fb, p->Ntot, nb, p->stream)))
-                return rc;
+            if (fft_is_pruned(p)) {
+                if ((rc = pruned_grid_fft<T>(p, fw, nb))) return rc;
+                tm.mark("fft");
+                if ((rc = pruned_deconv<T>(p, fw, fb, nb))) return rc;
+            } else {
+                if ((rc = fft_exec(p, fw, nb))) return rc;
+                tm.mark("fft");
+                if ((rc = launch_deconv<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2],
+                                           p->opts.modeord, fw, p->F, fb, p->Ntot, nb,
+                                           p->stream)))
+                    return rc;
+            }
             tm.mark("deconv");
         } else {
-            if ((rc = launch_amplify<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2],
-                                        p->opts.modeord, fb, p->Ntot, fw, p->F, nb, p->stream)))
-                return rc;
-            tm.mark("amplify");
-            if ((rc = fft_exec(p, fw, nb))) return rc;
-            tm.mark("fft");
+            if ((rc = modes_to_grid<T>(p, fb, fw, nb, tm))) return rc;
             if ((rc = launch_interp<T>(p->method, g, pts, bi, coef, kem, p->kp.beta, fw, p->F, cb,
                                        M, nb, p->stream)))
                 return rc;
@@ -886,13 +953,8 @@ static int interp_grad_t(jfa_plan p, const T *fk, const T *w, T *val, T *grad) {
         const int nb = std::min(p->batch, p->ntransf - t0);
         int rc = JFA_OK;
         if (fk) {
-            rc = launch_amplify<T>(p->dim, p->nmodes, g.nf, dk[0], dk[1], dk[2], p->opts.modeord,
-                                   fk + 2 * (int64_t)t0 * p->Ntot, p->Ntot, fw, p->F, nb,
-                                   p->stream);
-            if (rc) return rc;
-            tm.mark("amplify");
-            if ((rc = fft_exec(p, fw, nb))) return rc;
-            tm.mark("fft");
+            if ((rc = modes_to_grid<T>(p, fk + 2 * (int64_t)t0 * p->Ntot, fw, nb, tm)))
+                return rc;
         }
         rc = launch_interp_grad<T>(g, pts, bi, (const T *)p->d_coef, fw, p->F, nb,
                                    w ? w + 2 * (int64_t)t0 * M : nullptr, M,
@@ -967,6 +1029,8 @@ extern "C" int jfa_destroy(jfa_plan p) {
     if (p->used) (void)hipStreamSynchronize(p->stream);
     if (p->inner) jfa_destroy(p->inner);
     for (auto &kv : p->fft_plans) rocfft_plan_destroy(kv.second);
+    for (auto &kv : p->fft1_plans) rocfft_plan_destroy(kv.second);
+    free_ptr(p->d_fwt);
     if (p->fft_info) rocfft_execution_info_destroy(p->fft_info);
     free_ptr(p->d_fft_work);
     free_ptr(p->d_coef);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <map>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,9 @@ struct jfa_plan_s {
 
     // rocFFT
     std::map<int, rocfft_plan> fft_plans;  // by batch count
+    std::map<std::pair<int64_t, int64_t>, rocfft_plan> fft1_plans;  // 1D, by (length, batch)
+    void *d_fwt = nullptr;             // compact buffer of the pruned 2D / 3D FFT
+    size_t fwt_bytes = 0;
     rocfft_execution_info fft_info = nullptr;
     void *d_fft_work = nullptr;
     size_t fft_work_bytes = 0;
@@ -87,6 +91,20 @@ struct jfa_plan_s {
 namespace jfa {
 
 int fft_exec(jfa_plan p, void *data, int nbatch);
+// In-place 1D FFTs of `batch` contiguous rows of `len` complex values.
+int fft1d(jfa_plan p, void *data, int64_t len, int64_t batch);
+// Pruned FFT between the fine grid and the modes (fft_pruned.hip), 2D / 3D:
+// type 1 runs pruned_grid_fft then pruned_deconv, type 2 pruned_amplify then
+// pruned_modes_fft; fw holds nb transforms at stride F.
+bool fft_is_pruned(jfa_plan p);
+template <typename T>
+int pruned_grid_fft(jfa_plan p, T *fw, int nb);
+template <typename T>
+int pruned_deconv(jfa_plan p, T *fw, T *fk, int nb);
+template <typename T>
+int pruned_amplify(jfa_plan p, const T *fk, T *fw, int nb);
+template <typename T>
+int pruned_modes_fft(jfa_plan p, T *fw, int nb);
 bool strip_applicable(jfa_plan p);
 int setup_strip(jfa_plan p);
 int upload_kernel_tables(jfa_plan p, const int64_t *nmodes_for_dk);
