@@ -203,6 +203,8 @@ __global__ void __launch_bounds__(256)
     G[2 * l + 1] = vi;
 }
 
+}  // namespace
+
 template <typename T>
 int gather_cols(const T *A, T *B, int64_t outer, int64_t R, int64_t C, int64_t J, int64_t nsel,
                 int64_t cblk, hipStream_t s) {
@@ -232,6 +234,8 @@ int scatter_cols(const T *B, T *A, int64_t outer, int64_t R, int64_t C, int64_t 
     JFA_HIP_CHECK(hipGetLastError());
     return JFA_OK;
 }
+
+namespace {
 
 // complex elements per transform of the separate compact buffer: G (2D) or
 // G1 (3D)
@@ -351,6 +355,10 @@ int pruned_amplify(jfa_plan p, const T *fk, T *fw, int nb) {
 }
 
 #define JFA_INST(T)                                                 \
+    template int gather_cols<T>(const T *, T *, int64_t, int64_t, int64_t, int64_t, int64_t, \
+                                int64_t, hipStream_t);                                      \
+    template int scatter_cols<T>(const T *, T *, int64_t, int64_t, int64_t, int64_t, int64_t, \
+                                 int64_t, hipStream_t);                                      \
     template int pruned_grid_fft<T>(jfa_plan, T *, int);            \
     template int pruned_modes_fft<T>(jfa_plan, T *, int);           \
     template int pruned_deconv<T>(jfa_plan, T *, T *, int);         \

@@ -122,6 +122,7 @@ struct RaggedVirt {
     int64_t m;         // the problem's point count
     int64_t bin_off;   // the problem's first global bin
     int64_t fw_off;    // into the concatenated fine grids
+    int64_t g_off;     // into the concatenated pruned-FFT buffers ([n0][nf1] each)
     int64_t tile_off;  // into the concatenated scratch tiles
     int unit_off;      // first global unit
     int64_t n0, n1;    // modes, x fastest
@@ -241,6 +242,17 @@ int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *
 template <typename T>
 int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int ntr, T *pack,
                       hipStream_t stream);
+// Column gather / scatter transposes of the pruned FFT (fft_pruned.hip).  For
+// each of `outer` blocks: B[j][r] = A[r][col(j)] (j < J, r < R, A rows of C
+// complex values), col(j) = (j / nsel) * cblk + (i < (nsel+1)/2 ? i : cblk -
+// nsel + i) with i = j % nsel; scatter writes all of A, zeros where no j maps.
+template <typename T>
+int gather_cols(const T *A, T *B, int64_t outer, int64_t R, int64_t C, int64_t J, int64_t nsel,
+                int64_t cblk, hipStream_t s);
+template <typename T>
+int scatter_cols(const T *B, T *A, int64_t outer, int64_t R, int64_t C, int64_t J, int64_t nsel,
+                 int64_t cblk, hipStream_t s);
+
 // phases of launch_spread_strip (a bit mask): piece table, spreading, merge
 constexpr int kStripUnits = 1, kStripSpread = 2, kStripMerge = 4, kStripAll = 7;
 // Ragged batch: sort every problem's points at once (global bins), then the
@@ -253,7 +265,7 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
                         int kerevalmeth, double beta, T *fw_all, T *scratch_all, int phases,
                         hipStream_t stream);
 template <typename T>
-int launch_ragged_deconv(const RaggedLaunch &rl, const T *dk, int modeord, const T *fw_all,
+int launch_ragged_deconv(const RaggedLaunch &rl, const T *dk, int modeord, const T *g_all,
                          hipStream_t stream);
 template <typename T>
 int launch_interp(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,

@@ -102,13 +102,14 @@ __global__ void __launch_bounds__(256)
 }
 
 // Ragged batch: block b deconvolves 256 modes of the virtual problem v with
-// mode_blk[v] <= b; the output of problem i is f_i (pointer table entry
-// 3*nprob + i), transform-major.
+// mode_blk[v] <= b, reading its pruned-FFT buffer G[j0][y] (j0 the compact
+// column of k0, y = k1 mod nf1); the output of problem i is f_i (pointer table
+// entry 3*nprob + i), transform-major.
 template <typename T>
 __global__ void __launch_bounds__(256)
     ragged_deconv_kernel(const RaggedVirt *__restrict__ virt, const int64_t *__restrict__ mode_blk,
                          int nv, const T *__restrict__ dk, int modeord,
-                         const T *__restrict__ fw_all, const void *const *__restrict__ ptrs,
+                         const T *__restrict__ g_all, const void *const *__restrict__ ptrs,
                          int nprob) {
     int lo = 0, hi = nv;
     while (hi - lo > 1) {
@@ -128,21 +129,21 @@ __global__ void __launch_bounds__(256)
     const int64_t k0 = freq_of(q % n0, n0, modeord);
     const int64_t k1 = freq_of(q / n0, n1, modeord);
     const T s = dk0[k0 + n0 / 2] * dk1[k1 + n1 / 2];
-    const int64_t l = (k0 < 0 ? k0 + nf0 : k0) + nf0 * (k1 < 0 ? k1 + nf1 : k1);
-    const T *fw = fw_all + 2 * d.fw_off;
+    const int64_t l = (k0 < 0 ? k0 + n0 : k0) * nf1 + (k1 < 0 ? k1 + nf1 : k1);
+    const T *G = g_all + 2 * d.g_off;
     T *out = (T *)ptrs[3 * nprob + d.prob] + 2 * (int64_t)d.t * n0 * n1;
-    out[2 * q] = fw[2 * l] * s;
-    out[2 * q + 1] = fw[2 * l + 1] * s;
+    out[2 * q] = G[2 * l] * s;
+    out[2 * q + 1] = G[2 * l + 1] * s;
 }
 
 }  // namespace
 
 template <typename T>
-int launch_ragged_deconv(const RaggedLaunch &rl, const T *dk, int modeord, const T *fw_all,
+int launch_ragged_deconv(const RaggedLaunch &rl, const T *dk, int modeord, const T *g_all,
                          hipStream_t stream) {
     if (rl.mode_blocks == 0) return JFA_OK;
     ragged_deconv_kernel<T><<<(unsigned)rl.mode_blocks, 256, 0, stream>>>(
-        rl.virt, rl.mode_blk, rl.nv, dk, modeord, fw_all, rl.ptrs, rl.nprob);
+        rl.virt, rl.mode_blk, rl.nv, dk, modeord, g_all, rl.ptrs, rl.nprob);
     JFA_HIP_CHECK(hipGetLastError());
     return JFA_OK;
 }

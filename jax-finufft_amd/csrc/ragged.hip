@@ -8,8 +8,10 @@
 // (problem i owns [moff[i], moff[i+1])) and are sorted together: a point's key
 // is its bin within its problem offset by the problem's first global bin.  A
 // virtual problem is one (problem, transform) pair; virtual problems are
-// ordered by fine-grid size so that the fine grids of one size are contiguous
-// and transform together in one batched rocFFT call.  The spreading and merge
+// ordered by fine-grid size (then mode count n0) so that the fine grids of one
+// class are contiguous and transform together: one batched 1D rocFFT along x,
+// a gather of the n0 needed columns (transposed) into the class's pruned-FFT
+// buffer, one batched 1D rocFFT along y there.  The spreading and merge
 // kernels walk the global list of units of every virtual problem;
 // deconvolution walks the global list of 256-mode blocks.  The
 // device pointer table (x, y, c, f of every problem) is rewritten before each
@@ -64,11 +66,13 @@ struct jfa_ragged_s {
 
     // one batched rocFFT per fine-grid size
     struct FftClass {
-        int64_t nf0, nf1, fw_off;
+        int64_t nf0, nf1, n0, fw_off, g_off;
         int count;
-        rocfft_plan plan;
+        rocfft_plan plan_x, plan_y;
     };
     std::vector<FftClass> fft;
+    int64_t g_cells = 0;
+    void *d_g = nullptr;  // pruned-FFT buffers [n0][nf1] per virtual problem
     rocfft_execution_info fft_info = nullptr;
     void *d_fft_work = nullptr;
     size_t fft_work_bytes = 0;
@@ -109,13 +113,14 @@ void release(jfa_ragged r) {
                      (void **)&r->d_l0, &r->d_t, (void **)&r->d_key_in, (void **)&r->d_val_in,
                      (void **)&r->d_key_out, &r->d_sort_temp, (void **)&r->d_bin_start,
                      (void **)&r->d_piece_start, (void **)&r->d_npieces,
-                     (void **)&r->d_piece_unit, &r->d_fw, &r->d_scratch,
+                     (void **)&r->d_piece_unit, &r->d_fw, &r->d_scratch, &r->d_g,
                      &r->d_fft_work, (void **)&r->d_blocksum})
         free_dev(*q);
     if (r->h_ptrs) (void)hipHostFree((void *)r->h_ptrs);
     r->h_ptrs = nullptr;
     for (auto &c : r->fft)
-        if (c.plan) rocfft_plan_destroy(c.plan);
+        for (rocfft_plan pl : {c.plan_x, c.plan_y})
+            if (pl) rocfft_plan_destroy(pl);
     r->fft.clear();
     if (r->fft_info) rocfft_execution_info_destroy(r->fft_info);
     r->fft_info = nullptr;
@@ -195,7 +200,9 @@ int layout(jfa_ragged r) {
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         const GridGeom &ga = r->probs[a].g, &gb = r->probs[b].g;
-        return ga.nf[0] != gb.nf[0] ? ga.nf[0] < gb.nf[0] : ga.nf[1] < gb.nf[1];
+        if (ga.nf[0] != gb.nf[0]) return ga.nf[0] < gb.nf[0];
+        if (ga.nf[1] != gb.nf[1]) return ga.nf[1] < gb.nf[1];
+        return r->n0[a] < r->n0[b];
     });
     std::vector<int64_t> dk_off(np);
     int64_t dk = 0;
@@ -209,12 +216,13 @@ int layout(jfa_ragged r) {
     r->max_pieces = 0;
     r->mode_blk.assign(1, 0);
     r->fft.clear();
-    int64_t fw = 0, tiles = 0;
+    int64_t fw = 0, tiles = 0, gc = 0;
     for (int i : order) {
         const GridGeom &g = r->probs[i].g;
         const int64_t F = g.nf[0] * g.nf[1];
-        if (r->fft.empty() || r->fft.back().nf0 != g.nf[0] || r->fft.back().nf1 != g.nf[1])
-            r->fft.push_back({g.nf[0], g.nf[1], fw, 0, nullptr});
+        if (r->fft.empty() || r->fft.back().nf0 != g.nf[0] || r->fft.back().nf1 != g.nf[1] ||
+            r->fft.back().n0 != r->n0[i])
+            r->fft.push_back({g.nf[0], g.nf[1], r->n0[i], fw, gc, 0, nullptr, nullptr});
         for (int t = 0; t < r->ntr[i]; ++t) {
             RaggedVirt v{};
             v.sg = sg[i];
@@ -226,6 +234,7 @@ int layout(jfa_ragged r) {
             v.m = r->npts[i];
             v.bin_off = r->probs[i].bin_off;
             v.fw_off = fw;
+            v.g_off = gc;
             v.tile_off = tiles;
             v.unit_off = r->vunit_off.back();
             v.n0 = r->n0[i];
@@ -234,6 +243,7 @@ int layout(jfa_ragged r) {
             v.moff = r->moff[i];
             r->virt.push_back(v);
             fw += F;
+            gc += r->n0[i] * g.nf[1];
             tiles += strip_max_pieces(sg[i], r->npts[i]) * strip_tile_cells(sg[i]);
             r->max_pieces += strip_max_pieces(sg[i], r->npts[i]);
             r->unit_virt.insert(r->unit_virt.end(), sg[i].nunits, (int)r->virt.size() - 1);
@@ -243,6 +253,7 @@ int layout(jfa_ragged r) {
         }
     }
     r->fw_cells = fw;
+    r->g_cells = gc;
     r->tile_cells = tiles;
     if ((int64_t)r->vunit_off.back() > INT_MAX / 2) {
         set_error("ragged batch has too many spreading units");
@@ -321,23 +332,34 @@ int allocate(jfa_ragged r) {
         return JFA_ERR_ALLOC;
     }
 
-    // one batched in-place FFT per fine-grid size
+    if (hipMalloc(&r->d_g, 2 * real * std::max<int64_t>(r->g_cells, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("could not allocate the ragged batch's pruned-FFT buffers (%lld cells)",
+                  (long long)r->g_cells);
+        return JFA_ERR_ALLOC;
+    }
+    // per class: in-place 1D FFTs along x of every row, then along y of the
+    // n0 gathered columns
     for (auto &c : r->fft) {
-        size_t lengths[2] = {(size_t)c.nf0, (size_t)c.nf1};
-        rocfft_status st = rocfft_plan_create(
-            &c.plan, rocfft_placement_inplace,
-            r->iflag > 0 ? rocfft_transform_type_complex_inverse
-                         : rocfft_transform_type_complex_forward,
-            r->is_double ? rocfft_precision_double : rocfft_precision_single, 2, lengths,
-            (size_t)c.count, nullptr);
-        if (st != rocfft_status_success) {
-            c.plan = nullptr;
-            set_error("rocfft_plan_create failed (status %d)", (int)st);
-            return JFA_ERR_FFT;
+        const size_t len[2] = {(size_t)c.nf0, (size_t)c.nf1};
+        const size_t batch[2] = {(size_t)(c.count * c.nf1), (size_t)(c.count * c.n0)};
+        rocfft_plan *pl[2] = {&c.plan_x, &c.plan_y};
+        for (int a = 0; a < 2; ++a) {
+            rocfft_status st = rocfft_plan_create(
+                pl[a], rocfft_placement_inplace,
+                r->iflag > 0 ? rocfft_transform_type_complex_inverse
+                             : rocfft_transform_type_complex_forward,
+                r->is_double ? rocfft_precision_double : rocfft_precision_single, 1, &len[a],
+                batch[a], nullptr);
+            if (st != rocfft_status_success) {
+                *pl[a] = nullptr;
+                set_error("rocfft_plan_create failed (status %d)", (int)st);
+                return JFA_ERR_FFT;
+            }
+            size_t ws = 0;
+            rocfft_plan_get_work_buffer_size(*pl[a], &ws);
+            r->fft_work_bytes = std::max(r->fft_work_bytes, ws);
         }
-        size_t ws = 0;
-        rocfft_plan_get_work_buffer_size(c.plan, &ws);
-        r->fft_work_bytes = std::max(r->fft_work_bytes, ws);
     }
     if (r->fft_work_bytes) JFA_HIP_CHECK(hipMalloc(&r->d_fft_work, r->fft_work_bytes));
     if (rocfft_execution_info_create(&r->fft_info) != rocfft_status_success) {
@@ -394,15 +416,22 @@ int enqueue_execute(jfa_ragged r, hipStream_t s) {
         return rc;
     rocfft_execution_info_set_stream(r->fft_info, s);
     for (auto &c : r->fft) {
-        void *bufs[1] = {(T *)r->d_fw + 2 * c.fw_off};
-        rocfft_status st = rocfft_execute(c.plan, bufs, nullptr, r->fft_info);
+        T *fw = (T *)r->d_fw + 2 * c.fw_off;
+        T *G = (T *)r->d_g + 2 * c.g_off;
+        void *bx[1] = {fw}, *by[1] = {G};
+        rocfft_status st = rocfft_execute(c.plan_x, bx, nullptr, r->fft_info);
+        if (st == rocfft_status_success) {
+            if (int rc = gather_cols<T>(fw, G, c.count, c.nf1, c.nf0, c.n0, c.n0, c.nf0, s))
+                return rc;
+            st = rocfft_execute(c.plan_y, by, nullptr, r->fft_info);
+        }
         if (st != rocfft_status_success) {
             set_error("rocfft_execute failed (status %d)", (int)st);
             return JFA_ERR_FFT;
         }
     }
     return launch_ragged_deconv<T>(r->rl, (const T *)r->d_dk, r->opts.modeord,
-                                   (const T *)r->d_fw, s);
+                                   (const T *)r->d_g, s);
 }
 
 template <typename T>
