@@ -176,10 +176,11 @@ int layout(jfa_ragged r) {
                       (long long)kMaxFineCells);
             return JFA_ERR_TOO_LARGE;
         }
-        // many problems fill the GPU together: units of at least 16 rows (taller
-        // units sized by density, ~2048 points each, measured slower: 7.8 vs
-        // 6.8 ms on the 64-problem batch)
-        sg[i] = strip_geometry(g, r->opts.gpu_maxsubprobsize, 16);
+        // many problems fill the GPU together, so units can be taller than one
+        // plan's (less tile overlap).  Measured on the 64-problem benchmark
+        // batch: at least 16 rows 6.2 ms, 32 rows 5.8 ms, 64 rows 6.2 ms, and 16
+        // to 64 rows by density (one piece per unit) 6.1 ms.
+        sg[i] = strip_geometry(g, r->opts.gpu_maxsubprobsize, 32);
         r->probs[i].moff = r->moff[i];
         r->probs[i].bin_off = bins;
         r->moff[i + 1] = r->moff[i] + r->npts[i];
