@@ -128,79 +128,131 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// out[t][i] = G[t][...] * prod_d dk_d[k_d + n_d/2]; G is [n0][nf1] (2D) or
-// [n0][n1][nf2] (3D) per transform, the last axis at full length.
-template <typename T, int DIM>
-__global__ void __launch_bounds__(256)
-    deconv_pruned_kernel(int64_t n0, int64_t n1, int64_t n2, int64_t nf1, int64_t nf2,
-                         const T *__restrict__ dk0, const T *__restrict__ dk1,
-                         const T *__restrict__ dk2, int modeord, const T *__restrict__ G,
-                         int64_t g_stride, T *__restrict__ out, int64_t out_stride) {
-    const int64_t ntot = n0 * n1 * n2;
-    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= ntot) return;
-    G += 2 * g_stride * blockIdx.y;
-    out += 2 * out_stride * blockIdx.y;
-    const int64_t k0 = freq_of(q % n0, n0, modeord);
-    const int64_t k1 = freq_of((q / n0) % n1, n1, modeord);
-    T s = dk0[k0 + n0 / 2] * dk1[k1 + n1 / 2];
-    int64_t l;
-    if constexpr (DIM == 2) {
-        l = compact_of(k0, n0) * nf1 + (k1 < 0 ? k1 + nf1 : k1);
-    } else {
-        const int64_t k2 = freq_of(q / (n0 * n1), n2, modeord);
-        s *= dk2[k2 + n2 / 2];
-        l = (compact_of(k0, n0) * n1 + compact_of(k1, n1)) * nf2 + (k2 < 0 ? k2 + nf2 : k2);
-    }
-    out[2 * q] = G[2 * l] * s;
-    out[2 * q + 1] = G[2 * l + 1] * s;
+// The compact buffer of one transform is G[j0][jm][l]: j0 < n0 the compact
+// x column, jm < nm the compact y index (3D; nm = 1 in 2D), l < nfl the
+// full-length last axis (y in 2D, z in 3D).  Mode (i0, im, il) of the output
+// (i0 fastest; il the last axis' mode index) sits at G[j0(k0)][jm(km)][kl mod
+// nfl].  Both kernels below move one 32 x 32 tile of (x, last axis) through
+// LDS so that G is read / written along l and the modes along i0.
+struct PrunedShape {
+    int64_t n0, nm, nl;    // modes along x, the middle axis, the last axis
+    int64_t nfl;           // fine size of the last axis
+    int64_t t0, tl, nmid;  // tiles along x and the last axis, middle count
+};
+
+__device__ __forceinline__ int64_t idx_of(int64_t k, int64_t n, int modeord) {
+    return modeord == 0 ? k + n / 2 : (k < 0 ? k + n : k);
 }
 
-// G (layout as above) from the modes: mode value times 1/psi_hat where the
-// full-length last axis holds a mode frequency, 0 elsewhere; every element of
-// G is written.
-template <typename T, int DIM>
+// out[t][i0 + n0*(im + nm*il)] = G[...] * dk0 dkm dkl
+template <typename T>
 __global__ void __launch_bounds__(256)
-    amplify_pruned_kernel(int64_t n0, int64_t n1, int64_t n2, int64_t nf1, int64_t nf2,
-                          const T *__restrict__ dk0, const T *__restrict__ dk1,
-                          const T *__restrict__ dk2, int modeord, const T *__restrict__ f,
-                          int64_t f_stride, T *__restrict__ G, int64_t g_stride) {
-    const int64_t last = DIM == 2 ? nf1 : nf2;
-    const int64_t gsize = DIM == 2 ? n0 * nf1 : n0 * n1 * nf2;
-    const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (l >= gsize) return;
-    f += 2 * f_stride * blockIdx.y;
-    G += 2 * g_stride * blockIdx.y;
-    const int64_t c = l % last;  // fine index along the last axis
-    const int64_t nl = DIM == 2 ? n1 : n2;
-    const int64_t kl = c < (last + 1) / 2 ? c : c - last;
-    T vr = T(0), vi = T(0);
-    if (kl >= -(nl / 2) && kl <= (nl - 1) / 2) {
-        const int64_t rest = l / last;
-        int64_t j0, j1 = 0;
-        if constexpr (DIM == 2) {
-            j0 = rest;
-        } else {
-            j0 = rest / n1;
-            j1 = rest % n1;
-        }
-        const int64_t k0 = j0 < (n0 + 1) / 2 ? j0 : j0 - n0;
-        auto idx_of = [&](int64_t k, int64_t n) { return modeord == 0 ? k + n / 2 : (k < 0 ? k + n : k); };
-        T s = dk0[k0 + n0 / 2];
-        int64_t idx = idx_of(k0, n0);
-        if constexpr (DIM == 2) {
-            s *= dk1[kl + n1 / 2];
-            idx += n0 * idx_of(kl, n1);
-        } else {
-            const int64_t k1 = j1 < (n1 + 1) / 2 ? j1 : j1 - n1;
-            s *= dk1[k1 + n1 / 2] * dk2[kl + n2 / 2];
-            idx += n0 * (idx_of(k1, n1) + n1 * idx_of(kl, n2));
-        }
-        vr = f[2 * idx] * s;
-        vi = f[2 * idx + 1] * s;
+    deconv_pruned_kernel(PrunedShape sh, const T *__restrict__ dk0, const T *__restrict__ dkm,
+                         const T *__restrict__ dkl, int modeord, const T *__restrict__ G,
+                         int64_t g_stride, T *__restrict__ out, int64_t out_stride) {
+    __shared__ T tr[kTile][kTile + 1], ti[kTile][kTile + 1];
+    int64_t b = blockIdx.x;
+    const int64_t xb = b % sh.t0;
+    b /= sh.t0;
+    const int64_t lb = b % sh.tl;
+    b /= sh.tl;
+    const int64_t im = b % sh.nmid;
+    const int64_t t = b / sh.nmid;
+    G += 2 * g_stride * t;
+    out += 2 * out_stride * t;
+    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+    const int64_t km = sh.nm > 1 ? freq_of(im, sh.nm, modeord) : 0;
+    const int64_t jm = compact_of(km, sh.nm);
+    // read along the last axis
+    const int64_t il = lb * kTile + tx;
+    int64_t l = -1;
+    if (il < sh.nl) {
+        const int64_t kl = freq_of(il, sh.nl, modeord);
+        l = kl < 0 ? kl + sh.nfl : kl;
     }
-    G[2 * l] = vr;
-    G[2 * l + 1] = vi;
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t i0 = xb * kTile + k;
+        T vr = T(0), vi = T(0);
+        if (l >= 0 && i0 < sh.n0) {
+            const int64_t j0 = compact_of(freq_of(i0, sh.n0, modeord), sh.n0);
+            const int64_t g = (j0 * sh.nm + jm) * sh.nfl + l;
+            vr = G[2 * g];
+            vi = G[2 * g + 1];
+        }
+        tr[k][tx] = vr;
+        ti[k][tx] = vi;
+    }
+    __syncthreads();
+    // write along x
+    const int64_t i0 = xb * kTile + tx;
+    if (i0 >= sh.n0) return;
+    const int64_t k0 = freq_of(i0, sh.n0, modeord);
+    const T s0 = dk0[k0 + sh.n0 / 2] * (sh.nm > 1 ? dkm[km + sh.nm / 2] : T(1));
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t ill = lb * kTile + k;
+        if (ill >= sh.nl) break;
+        const T s = s0 * dkl[freq_of(ill, sh.nl, modeord) + sh.nl / 2];
+        const int64_t q = i0 + sh.n0 * (im + sh.nm * ill);
+        out[2 * q] = tr[tx][k] * s;
+        out[2 * q + 1] = ti[tx][k] * s;
+    }
+}
+
+// G[t][j0][jm][l] for every l < nfl: the mode value times 1/psi_hat where l
+// holds a mode frequency, 0 elsewhere (every element of G is written)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    amplify_pruned_kernel(PrunedShape sh, const T *__restrict__ dk0, const T *__restrict__ dkm,
+                          const T *__restrict__ dkl, int modeord, const T *__restrict__ f,
+                          int64_t f_stride, T *__restrict__ G, int64_t g_stride) {
+    __shared__ T tr[kTile][kTile + 1], ti[kTile][kTile + 1];
+    int64_t b = blockIdx.x;
+    const int64_t xb = b % sh.t0;  // tiles of compact x columns
+    b /= sh.t0;
+    const int64_t lb = b % sh.tl;  // tiles of the full last axis
+    b /= sh.tl;
+    const int64_t jm = b % sh.nmid;
+    const int64_t t = b / sh.nmid;
+    f += 2 * f_stride * t;
+    G += 2 * g_stride * t;
+    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+    const int64_t km = sh.nm > 1 ? (jm < (sh.nm + 1) / 2 ? jm : jm - sh.nm) : 0;
+    // read the modes along x
+    const int64_t j0 = xb * kTile + tx;
+    int64_t i0 = -1;
+    T s0 = T(0);
+    if (j0 < sh.n0) {
+        const int64_t k0 = j0 < (sh.n0 + 1) / 2 ? j0 : j0 - sh.n0;
+        i0 = idx_of(k0, sh.n0, modeord);
+        s0 = dk0[k0 + sh.n0 / 2] * (sh.nm > 1 ? dkm[km + sh.nm / 2] : T(1));
+    }
+    const int64_t im = sh.nm > 1 ? idx_of(km, sh.nm, modeord) : 0;
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t l = lb * kTile + k;
+        T vr = T(0), vi = T(0);
+        if (i0 >= 0 && l < sh.nfl) {
+            const int64_t kl = l < (sh.nfl + 1) / 2 ? l : l - sh.nfl;
+            if (kl >= -(sh.nl / 2) && kl <= (sh.nl - 1) / 2) {
+                const int64_t q = i0 + sh.n0 * (im + sh.nm * idx_of(kl, sh.nl, modeord));
+                const T s = s0 * dkl[kl + sh.nl / 2];
+                vr = f[2 * q] * s;
+                vi = f[2 * q + 1] * s;
+            }
+        }
+        tr[k][tx] = vr;
+        ti[k][tx] = vi;
+    }
+    __syncthreads();
+    // write G along the last axis
+    const int64_t l = lb * kTile + tx;
+    if (l >= sh.nfl) return;
+    for (int k = ty; k < kTile; k += 8) {
+        const int64_t jj = xb * kTile + k;
+        if (jj >= sh.n0) break;
+        const int64_t g = (jj * sh.nm + jm) * sh.nfl + l;
+        G[2 * g] = tr[tx][k];
+        G[2 * g + 1] = ti[tx][k];
+    }
 }
 
 }  // namespace
@@ -314,22 +366,34 @@ int pruned_modes_fft(jfa_plan p, T *fw, int nb) {
     return fft1d(p, fw, nf[0], (int64_t)nb * nf[1] * nf[2]);
 }
 
+// Shape of the compact buffer and the tiling of the deconvolution /
+// amplification kernels; g_stride = complex elements per transform.
+static PrunedShape pruned_shape(jfa_plan p, bool over_fine_last, int64_t *g_stride) {
+    const int64_t *n = p->nmodes, *nf = p->geom.nf;
+    PrunedShape sh;
+    sh.n0 = n[0];
+    sh.nm = p->dim == 3 ? n[1] : 1;
+    sh.nl = p->dim == 3 ? n[2] : n[1];
+    sh.nfl = p->dim == 3 ? nf[2] : nf[1];
+    sh.nmid = sh.nm;
+    sh.t0 = (sh.n0 + kTile - 1) / kTile;
+    sh.tl = ((over_fine_last ? sh.nfl : sh.nl) + kTile - 1) / kTile;
+    *g_stride = sh.n0 * sh.nm * sh.nfl;
+    return sh;
+}
+
 template <typename T>
 int pruned_deconv(jfa_plan p, T *fw, T *fk, int nb) {
-    const int64_t *n = p->nmodes, *nf = p->geom.nf;
-    const int64_t ntot = n[0] * n[1] * n[2];
-    if (ntot == 0 || nb == 0) return JFA_OK;
+    int64_t gs;
+    const PrunedShape sh = pruned_shape(p, false, &gs);
+    const int64_t blocks = sh.t0 * sh.tl * sh.nmid * nb;
+    if (blocks == 0) return JFA_OK;
     const T *G = compact_modes<T>(p, fw);
-    const int64_t gs = p->dim == 2 ? fwt_cells(p) : n[0] * n[1] * nf[2];
     const T *dk[3] = {(const T *)p->d_dk[0], (const T *)p->d_dk[1], (const T *)p->d_dk[2]};
-    dim3 grid((unsigned)((ntot + 255) / 256), nb);
-    if (p->dim == 2)
-        deconv_pruned_kernel<T, 2><<<grid, 256, 0, p->stream>>>(
-            n[0], n[1], 1, nf[1], 1, dk[0], dk[1], dk[2], p->opts.modeord, G, gs, fk, p->Ntot);
-    else
-        deconv_pruned_kernel<T, 3><<<grid, 256, 0, p->stream>>>(
-            n[0], n[1], n[2], nf[1], nf[2], dk[0], dk[1], dk[2], p->opts.modeord, G, gs, fk,
-            p->Ntot);
+    const T *dkm = p->dim == 3 ? dk[1] : dk[0];
+    const T *dkl = p->dim == 3 ? dk[2] : dk[1];
+    deconv_pruned_kernel<T><<<(unsigned)blocks, 256, 0, p->stream>>>(
+        sh, dk[0], dkm, dkl, p->opts.modeord, G, gs, fk, p->Ntot);
     JFA_HIP_CHECK(hipGetLastError());
     return JFA_OK;
 }
@@ -337,19 +401,16 @@ int pruned_deconv(jfa_plan p, T *fw, T *fk, int nb) {
 template <typename T>
 int pruned_amplify(jfa_plan p, const T *fk, T *fw, int nb) {
     if (int rc = reserve_fwt(p)) return rc;
-    const int64_t *n = p->nmodes, *nf = p->geom.nf;
+    int64_t gs;
+    const PrunedShape sh = pruned_shape(p, true, &gs);
+    const int64_t blocks = sh.t0 * sh.tl * sh.nmid * nb;
+    if (blocks == 0) return JFA_OK;
     T *G = compact_modes<T>(p, fw);
-    const int64_t gs = p->dim == 2 ? fwt_cells(p) : n[0] * n[1] * nf[2];
-    if (gs == 0 || nb == 0) return JFA_OK;
     const T *dk[3] = {(const T *)p->d_dk[0], (const T *)p->d_dk[1], (const T *)p->d_dk[2]};
-    dim3 grid((unsigned)((gs + 255) / 256), nb);
-    if (p->dim == 2)
-        amplify_pruned_kernel<T, 2><<<grid, 256, 0, p->stream>>>(
-            n[0], n[1], 1, nf[1], 1, dk[0], dk[1], dk[2], p->opts.modeord, fk, p->Ntot, G, gs);
-    else
-        amplify_pruned_kernel<T, 3><<<grid, 256, 0, p->stream>>>(
-            n[0], n[1], n[2], nf[1], nf[2], dk[0], dk[1], dk[2], p->opts.modeord, fk, p->Ntot,
-            G, gs);
+    const T *dkm = p->dim == 3 ? dk[1] : dk[0];
+    const T *dkl = p->dim == 3 ? dk[2] : dk[1];
+    amplify_pruned_kernel<T><<<(unsigned)blocks, 256, 0, p->stream>>>(
+        sh, dk[0], dkm, dkl, p->opts.modeord, fk, p->Ntot, G, gs);
     JFA_HIP_CHECK(hipGetLastError());
     return JFA_OK;
 }

@@ -136,7 +136,7 @@ struct RaggedLaunch {
     const RaggedVirt *virt;
     const int64_t *moff;       // nprob + 1
     const int *vunit_off;      // nv + 1
-    const int64_t *mode_blk;   // nv + 1: first deconv block (256 modes)
+    const int64_t *mode_blk;   // nv + 1: first deconv block (32 x 32 modes)
     const void *const *ptrs;   // x[nprob], y[nprob], c[nprob], f[nprob]
     const int *unit_virt;      // nunits: virtual problem of every global unit
     int *bin_start, *piece_start, *npieces;

@@ -101,16 +101,18 @@ __global__ void __launch_bounds__(256)
     fw[2 * l + 1] = vi;
 }
 
-// Ragged batch: block b deconvolves 256 modes of the virtual problem v with
-// mode_blk[v] <= b, reading its pruned-FFT buffer G[j0][y] (j0 the compact
-// column of k0, y = k1 mod nf1); the output of problem i is f_i (pointer table
-// entry 3*nprob + i), transform-major.
+// Ragged batch: block b deconvolves a 32 x 32 tile of modes of the virtual
+// problem v with mode_blk[v] <= b < mode_blk[v + 1], reading its pruned-FFT
+// buffer G[j0][y] (j0 the compact column of k0, y = k1 mod nf1) along y and
+// writing the modes along x through LDS; the output of problem i is f_i
+// (pointer table entry 3*nprob + i), transform-major.
 template <typename T>
 __global__ void __launch_bounds__(256)
     ragged_deconv_kernel(const RaggedVirt *__restrict__ virt, const int64_t *__restrict__ mode_blk,
                          int nv, const T *__restrict__ dk, int modeord,
                          const T *__restrict__ g_all, const void *const *__restrict__ ptrs,
                          int nprob) {
+    __shared__ T tr[32][33], ti[32][33];
     int lo = 0, hi = nv;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -120,20 +122,47 @@ __global__ void __launch_bounds__(256)
             hi = mid;
     }
     const RaggedVirt d = virt[lo];
-    const int64_t q = ((int64_t)blockIdx.x - mode_blk[lo]) * 256 + threadIdx.x;
-    const int64_t n0 = d.n0, n1 = d.n1;
-    if (q >= n0 * n1) return;
-    const int64_t nf0 = d.sg.nf0, nf1 = d.sg.nf1;
+    const int64_t n0 = d.n0, n1 = d.n1, nf1 = d.sg.nf1;
+    const int64_t tiles0 = (n0 + 31) / 32;
+    const int64_t tb = (int64_t)blockIdx.x - mode_blk[lo];
+    const int64_t xb = tb % tiles0, yb = tb / tiles0;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const T *G = g_all + 2 * d.g_off;
+    // read along y
+    const int64_t i1 = yb * 32 + tx;
+    int64_t y = -1;
+    if (i1 < n1) {
+        const int64_t k1 = freq_of(i1, n1, modeord);
+        y = k1 < 0 ? k1 + nf1 : k1;
+    }
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t i0 = xb * 32 + k;
+        T vr = T(0), vi = T(0);
+        if (y >= 0 && i0 < n0) {
+            const int64_t k0 = freq_of(i0, n0, modeord);
+            const int64_t l = (k0 < 0 ? k0 + n0 : k0) * nf1 + y;
+            vr = G[2 * l];
+            vi = G[2 * l + 1];
+        }
+        tr[k][tx] = vr;
+        ti[k][tx] = vi;
+    }
+    __syncthreads();
+    // write along x
+    const int64_t i0 = xb * 32 + tx;
+    if (i0 >= n0) return;
     const T *dk0 = dk + d.dk_off;
     const T *dk1 = dk0 + n0;
-    const int64_t k0 = freq_of(q % n0, n0, modeord);
-    const int64_t k1 = freq_of(q / n0, n1, modeord);
-    const T s = dk0[k0 + n0 / 2] * dk1[k1 + n1 / 2];
-    const int64_t l = (k0 < 0 ? k0 + n0 : k0) * nf1 + (k1 < 0 ? k1 + nf1 : k1);
-    const T *G = g_all + 2 * d.g_off;
+    const T s0 = dk0[freq_of(i0, n0, modeord) + n0 / 2];
     T *out = (T *)ptrs[3 * nprob + d.prob] + 2 * (int64_t)d.t * n0 * n1;
-    out[2 * q] = G[2 * l] * s;
-    out[2 * q + 1] = G[2 * l + 1] * s;
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t ii = yb * 32 + k;
+        if (ii >= n1) break;
+        const T s = s0 * dk1[freq_of(ii, n1, modeord) + n1 / 2];
+        const int64_t q = i0 + n0 * ii;
+        out[2 * q] = tr[tx][k] * s;
+        out[2 * q + 1] = ti[tx][k] * s;
+    }
 }
 
 }  // namespace

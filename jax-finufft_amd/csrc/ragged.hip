@@ -249,7 +249,7 @@ int layout(jfa_ragged r) {
             r->max_pieces += strip_max_pieces(sg[i], r->npts[i]);
             r->unit_virt.insert(r->unit_virt.end(), sg[i].nunits, (int)r->virt.size() - 1);
             r->vunit_off.push_back(r->vunit_off.back() + sg[i].nunits);
-            r->mode_blk.push_back(r->mode_blk.back() + (v.n0 * v.n1 + 255) / 256);
+            r->mode_blk.push_back(r->mode_blk.back() + ((v.n0 + 31) / 32) * ((v.n1 + 31) / 32));
             r->fft.back().count++;
         }
     }
