@@ -177,10 +177,19 @@ int layout(jfa_ragged r) {
             return JFA_ERR_TOO_LARGE;
         }
         // many problems fill the GPU together, so units can be taller than one
-        // plan's (less tile overlap).  Measured on the 64-problem benchmark
-        // batch: at least 16 rows 6.2 ms, 32 rows 5.8 ms, 64 rows 6.2 ms, and 16
-        // to 64 rows by density (one piece per unit) 6.1 ms.
-        sg[i] = strip_geometry(g, r->opts.gpu_maxsubprobsize, 32);
+        // plan's: 32 rows (less tile overlap) when such a unit holds at most
+        // pmax points on average, else 16 (every piece writes a whole tile, so
+        // dense problems keep short units).  Measured on the 64-problem
+        // benchmark batch: 16 rows everywhere 6.2 ms, 32 rows 5.8 ms, 64 rows
+        // 6.2 ms.
+        int min_h = 16;
+        {
+            const int pmax = r->opts.gpu_maxsubprobsize > 0 ? r->opts.gpu_maxsubprobsize : 4096;
+            const double per_unit =
+                (double)r->npts[i] * (65 - ns) * 32 / (double)(g.nf[0] * g.nf[1]);
+            if (per_unit <= pmax) min_h = 32;
+        }
+        sg[i] = strip_geometry(g, r->opts.gpu_maxsubprobsize, min_h);
         r->probs[i].moff = r->moff[i];
         r->probs[i].bin_off = bins;
         r->moff[i + 1] = r->moff[i] + r->npts[i];
