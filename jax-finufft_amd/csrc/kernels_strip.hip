@@ -415,33 +415,45 @@ __device__ __forceinline__ void merge_unit(const StripGeom &sg, const int *__res
     const int rows_tile = sg.H + NS - 1;
     const int64_t tile_cells = (int64_t)rows_tile * kStripLanes;
     const int base = piece_start[0];
-    int cs[3], co[3], ck[3], ro[3];
-    cs[0] = s;
-    co[0] = 0;
-    ck[0] = k;
-    ro[0] = 0;
+    int cs[3] = {s, 0, 0}, co[3] = {0, 0, 0}, ck[3] = {k, 0, 0}, ro[3] = {0, 0, 0};
     const int ncs = 1 + left_windows(s, SW, kStripLanes, sg.S, sg.nf0, cs + 1, co + 1);
     const int nck = 1 + left_windows(k, sg.H, rows_tile, sg.K, sg.nf1, ck + 1, ro + 1);
+    // the covering units' piece ranges, once per workgroup (uniform); entry
+    // 3 i + j is strip candidate i with unit-row candidate j (no pieces when
+    // either does not exist)
+    int pfirst[9], pcount[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            pfirst[3 * i + j] = 0;
+            pcount[3 * i + j] = 0;
+            if (i < ncs && j < nck) {
+                const int u2 = cs[i] * sg.K + ck[j];
+                pfirst[3 * i + j] = piece_start[u2] - base;
+                pcount[3 * i + j] = piece_start[u2 + 1] - piece_start[u2];
+            }
+        }
     const int c = threadIdx.x & (kStripLanes - 1);
     if (c >= wcols) return;
+    const Pair<T> *tiles = reinterpret_cast<const Pair<T> *>(scratch);
     for (int r = threadIdx.x / kStripLanes; r < wrows; r += blockDim.x / kStripLanes) {
         T ar = T(0), ai = T(0);
-        for (int i = 0; i < ncs; ++i) {
-            const int tc = c + co[i];
-            if (tc >= kStripLanes) continue;
-            for (int j = 0; j < nck; ++j) {
-                const int tr = r + ro[j];
-                if (tr >= rows_tile) continue;
-                const int u2 = cs[i] * sg.K + ck[j];
-                const int64_t off = (int64_t)tr * kStripLanes + tc;
-                for (int pc = piece_start[u2], pend = piece_start[u2 + 1]; pc < pend; ++pc) {
-                    const Pair<T> v =
-                        reinterpret_cast<const Pair<T> *>(scratch)[(pc - base) * tile_cells + off];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int q = 3 * i + j;
+                const int tc = c + co[i], tr = r + ro[j];
+                if (pcount[q] == 0 || tc >= kStripLanes || tr >= rows_tile) continue;
+                const Pair<T> *src =
+                    tiles + (int64_t)pfirst[q] * tile_cells + tr * kStripLanes + tc;
+                for (int pc = 0; pc < pcount[q]; ++pc) {
+                    const Pair<T> v = src[pc * tile_cells];
                     ar += v.x;
                     ai += v.y;
                 }
             }
-        }
         reinterpret_cast<Pair<T> *>(fw)[(int64_t)(y0 + r) * sg.nf0 + x0 + c] = Pair<T>{ar, ai};
     }
 }
