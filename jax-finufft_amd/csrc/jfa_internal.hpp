@@ -97,6 +97,8 @@ inline int64_t strip_tile_cells(const StripGeom &sg) { return (int64_t)(sg.H + s
 inline int64_t strip_max_pieces(const StripGeom &sg, int64_t M) {
     return sg.nunits + M / sg.pmax + 1;
 }
+// Units with more than one piece hold more than pmax points each.
+inline int64_t strip_max_heavy(const StripGeom &sg, int64_t M) { return M / (sg.pmax + 1) + 1; }
 
 // Strip geometry for a 2D grid g (nf and ns set): units of H rows sized for
 // about 4096 units, at least min_h rows, H in [ns, nf1 - ns + 1] so that a
@@ -141,6 +143,9 @@ struct RaggedLaunch {
     const int *unit_virt;      // nunits: virtual problem of every global unit
     int *bin_start, *piece_start, *npieces;
     int *piece_unit;           // global unit of every piece
+    int *heavy;                // [count, units with more than one piece ...]
+    int64_t max_heavy;         // most such units
+    int reduce_chunks;         // 256-cell chunks of the largest tile
     int *blocksum;             // 2 * ceil(nunits / 256) + 1 ints of piece-scan scratch
     int nprob, nv, ns, pmax, keyshift;
     int64_t mtot, nbins, nunits, mode_blocks;
@@ -234,8 +239,8 @@ int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const 
                   T *fw, int64_t fw_stride, int ntr, hipStream_t stream);
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
-                        int *piece_start, int *npieces, int *piece_unit, const T *coef,
-                        int kerevalmeth, double beta, int64_t pack_stride, T *fw,
+                        int *piece_start, int *npieces, int *piece_unit, int *heavy,
+                        const T *coef, int kerevalmeth, double beta, int64_t pack_stride, T *fw,
                         int64_t fw_stride, T *scratch, int64_t scratch_stride, int ntr,
                         int phases, hipStream_t stream);
 // pack[t][j] = (c[t][j], rec[j]) for t < ntr, j < M (records of 4 reals)

@@ -431,7 +431,8 @@ __device__ __forceinline__ void merge_unit(const StripGeom &sg, const int *__res
             if (i < ncs && j < nck) {
                 const int u2 = cs[i] * sg.K + ck[j];
                 pfirst[3 * i + j] = piece_start[u2] - base;
-                pcount[3 * i + j] = piece_start[u2 + 1] - piece_start[u2];
+                // a crowded unit's pieces were summed into its first tile
+                pcount[3 * i + j] = min(piece_start[u2 + 1] - piece_start[u2], 1);
             }
         }
     const int c = threadIdx.x & (kStripLanes - 1);
@@ -468,12 +469,14 @@ __device__ __forceinline__ int unit_pieces(const StripGeom &sg, const int *__res
 }
 
 // One workgroup: exclusive prefix of pieces(u) over u in [0, n); total at
-// piece_start[n] and in *npieces; with piece_unit, also the unit of every piece.
+// piece_start[n] and in *npieces; with piece_unit, also the unit of every piece;
+// with heavy, the list [count, u ...] of the units with more than one piece.
 template <typename PiecesOf>
 __device__ __forceinline__ void piece_scan(int n, PiecesOf pieces, int *piece_start,
-                                           int *npieces, int *piece_unit) {
+                                           int *npieces, int *piece_unit, int *heavy) {
     __shared__ int sa[1024];
     const int tid = threadIdx.x;
+    if (heavy && tid == 0) heavy[0] = 0;  // ordered before the atomics by the barriers below
     const int per = (n + 1023) / 1024;
     const int lo = min(tid * per, n);
     const int hi = min(lo + per, n);
@@ -493,6 +496,7 @@ __device__ __forceinline__ void piece_scan(int n, PiecesOf pieces, int *piece_st
         const int np = pieces(u);
         if (piece_unit)
             for (int i = 0; i < np; ++i) piece_unit[e + i] = u;
+        if (heavy && np > 1) heavy[1 + atomicAdd(&heavy[0], 1)] = u;
         e += np;
     }
     if (tid == 1023) {
@@ -546,10 +550,38 @@ __global__ void __launch_bounds__(256)
 
 __global__ void __launch_bounds__(1024)
     strip_units_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
-                       int *npieces, int *piece_unit) {
+                       int *npieces, int *piece_unit, int *heavy) {
     piece_scan(
         sg.nunits, [&](int u) { return unit_pieces(sg, bin_start, u); }, piece_start, npieces,
-        piece_unit);
+        piece_unit, heavy);
+}
+
+// Sum the tiles of every piece of a crowded unit into its first piece's tile
+// (in piece order), so that the merge reads one tile per unit; work items
+// (unit, 256-cell chunk of the tile) over a fixed grid.
+template <typename T, int NS>
+__global__ void __launch_bounds__(256)
+    strip_reduce_kernel(StripGeom sg, const int *__restrict__ piece_start,
+                        const int *__restrict__ heavy, T *__restrict__ scratch,
+                        int64_t scratch_stride) {
+    scratch += 2 * scratch_stride * blockIdx.y;
+    const int64_t tile_cells = (int64_t)(sg.H + NS - 1) * kStripLanes;
+    const int chunks = (int)((tile_cells + 255) / 256);
+    const int items = heavy[0] * chunks;
+    Pair<T> *tiles = reinterpret_cast<Pair<T> *>(scratch);
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        const int u = heavy[1 + w / chunks];
+        const int64_t cell = (int64_t)(w % chunks) * 256 + threadIdx.x;
+        if (cell >= tile_cells) continue;
+        const int first = piece_start[u], cnt = piece_start[u + 1] - first;
+        Pair<T> acc = tiles[first * tile_cells + cell];
+        for (int pc = 1; pc < cnt; ++pc) {
+            const Pair<T> v = tiles[(first + pc) * tile_cells + cell];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        tiles[first * tile_cells + cell] = acc;
+    }
 }
 
 // ------------------------------------------------------------ ragged batch
@@ -607,16 +639,17 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(1024)
     ragged_units_blocks_kernel(const int *__restrict__ vunit_off, int nv, int nblk,
                                const int *__restrict__ blocksum, int *blockoff,
-                               int *piece_start, int *npieces) {
+                               int *piece_start, int *npieces, int *heavy) {
     piece_scan(
-        nblk, [&](int b) { return blocksum[b]; }, blockoff, npieces, nullptr);
+        nblk, [&](int b) { return blocksum[b]; }, blockoff, npieces, nullptr, nullptr);
+    if (threadIdx.x == 0) heavy[0] = 0;  // the scan kernel that follows appends to it
     if (threadIdx.x == 1023) piece_start[vunit_off[nv]] = blockoff[nblk];
 }
 
 __global__ void __launch_bounds__(256)
     ragged_units_scan_kernel(const int *__restrict__ vunit_off, int nv,
                              const int *__restrict__ blockoff, int *piece_start,
-                             int *piece_unit) {
+                             int *piece_unit, int *heavy) {
     const int nu = vunit_off[nv];
     const int gu = blockIdx.x * 256 + threadIdx.x;
     const int c = gu < nu ? piece_start[gu] : 0;
@@ -626,6 +659,7 @@ __global__ void __launch_bounds__(256)
         const int first = blockoff[blockIdx.x] + e;
         piece_start[gu] = first;
         for (int i = 0; i < c; ++i) piece_unit[first + i] = gu;
+        if (c > 1) heavy[1 + atomicAdd(&heavy[0], 1)] = gu;
     }
 }
 
@@ -658,6 +692,32 @@ __global__ void __launch_bounds__(kStripLanes)
     }
 }
 
+// Ragged form of strip_reduce_kernel: heavy lists global units.
+template <typename T, int NS>
+__global__ void __launch_bounds__(256)
+    ragged_reduce_kernel(const RaggedVirt *__restrict__ virt, const int *__restrict__ unit_virt,
+                         const int *__restrict__ piece_start, const int *__restrict__ heavy,
+                         int chunks, T *__restrict__ scratch_all) {
+    const int items = heavy[0] * chunks;
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        const int gu = heavy[1 + w / chunks];
+        const RaggedVirt &d = virt[unit_virt[gu]];
+        const int64_t tile_cells = (int64_t)(d.sg.H + NS - 1) * kStripLanes;
+        const int64_t cell = (int64_t)(w % chunks) * 256 + threadIdx.x;
+        if (cell >= tile_cells) continue;
+        Pair<T> *tiles = reinterpret_cast<Pair<T> *>(scratch_all + 2 * d.tile_off);
+        const int first = piece_start[gu] - piece_start[d.unit_off];
+        const int cnt = piece_start[gu + 1] - piece_start[gu];
+        Pair<T> acc = tiles[first * tile_cells + cell];
+        for (int pc = 1; pc < cnt; ++pc) {
+            const Pair<T> v = tiles[(first + pc) * tile_cells + cell];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        tiles[first * tile_cells + cell] = acc;
+    }
+}
+
 // block b merges global unit b (unit b - unit_off of its virtual problem)
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
@@ -675,14 +735,14 @@ __global__ void __launch_bounds__(256)
 
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
-                        int *piece_start, int *npieces, int *piece_unit, const T *coef,
-                        int kerevalmeth, double beta, int64_t pack_stride, T *fw,
+                        int *piece_start, int *npieces, int *piece_unit, int *heavy,
+                        const T *coef, int kerevalmeth, double beta, int64_t pack_stride, T *fw,
                         int64_t fw_stride, T *scratch, int64_t scratch_stride, int ntr,
                         int phases, hipStream_t stream) {
     if (ntr == 0 || pts.M == 0) return JFA_OK;
     if (phases & kStripUnits) {
         strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces,
-                                                   piece_unit);
+                                                   piece_unit, heavy);
         JFA_HIP_CHECK(hipGetLastError());
     }
     const int64_t ub = sg.nunits + pts.M / sg.pmax + 1;
@@ -693,6 +753,13 @@ int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *
             spread_strip_kernel<T, NS_><<<dim3(grid, ntr), kStripLanes, 0, stream>>>(
                 sg, pts, bin_start, piece_start, npieces, piece_unit, coef, b, kerevalmeth,
                 pack_stride, scratch, scratch_stride);
+        if (phases & kStripMerge) {
+            const int64_t rb = strip_max_heavy(sg, pts.M) *
+                               ((strip_tile_cells(sg) + 255) / 256);
+            strip_reduce_kernel<T, NS_>
+                <<<dim3((unsigned)(rb < 2048 ? rb : 2048), ntr), 256, 0, stream>>>(
+                    sg, piece_start, heavy, scratch, scratch_stride);
+        }
         if (phases & kStripMerge)
             strip_merge_kernel<T, NS_><<<dim3((unsigned)sg.nunits, ntr), 256, 0, stream>>>(
                     sg, piece_start, scratch, scratch_stride, fw, fw_stride);
@@ -715,9 +782,10 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
         int *blockoff = rl.blocksum + nblk;  // nblk + 1 entries
         ragged_units_blocks_kernel<<<1, 1024, 0, stream>>>(rl.vunit_off, rl.nv, (int)nblk,
                                                            rl.blocksum, blockoff,
-                                                           rl.piece_start, rl.npieces);
+                                                           rl.piece_start, rl.npieces, rl.heavy);
         ragged_units_scan_kernel<<<nblk, 256, 0, stream>>>(rl.vunit_off, rl.nv, blockoff,
-                                                           rl.piece_start, rl.piece_unit);
+                                                           rl.piece_start, rl.piece_unit,
+                                                           rl.heavy);
         JFA_HIP_CHECK(hipGetLastError());
     }
     const int64_t ub = rl.nunits + pts.M / rl.pmax + rl.nv;
@@ -728,6 +796,11 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
             ragged_strip_kernel<T, NS_><<<grid, kStripLanes, 0, stream>>>(
                 rl.virt, rl.unit_virt, pts, rl.bin_start, rl.piece_start, rl.npieces,
                 rl.piece_unit, coef, b, kerevalmeth, rl.ptrs, rl.nprob, scratch_all);
+        if (phases & kStripMerge) {
+            const int64_t rb = rl.max_heavy * rl.reduce_chunks;
+            ragged_reduce_kernel<T, NS_><<<(unsigned)(rb < 2048 ? rb : 2048), 256, 0, stream>>>(
+                rl.virt, rl.unit_virt, rl.piece_start, rl.heavy, rl.reduce_chunks, scratch_all);
+        }
         if (phases & kStripMerge)
             ragged_merge_kernel<T, NS_><<<(unsigned)rl.nunits, 256, 0, stream>>>(
                 rl.virt, rl.unit_virt, rl.piece_start, scratch_all, fw_all);
@@ -748,8 +821,9 @@ int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int
 
 #define JFA_INST(T)                                                                           \
     template int launch_spread_strip<T>(const StripGeom &, const StripPts<T> &, const int *, \
-                                        int *, int *, int *, const T *, int, double, int64_t, \
-                                        T *, int64_t, T *, int64_t, int, int, hipStream_t);   \
+                                        int *, int *, int *, int *, const T *, int, double,   \
+                                        int64_t, T *, int64_t, T *, int64_t, int, int,        \
+                                        hipStream_t);                                         \
     template int launch_strip_pack<T>(const T *, int64_t, const T *, int64_t, int, T *,       \
                                       hipStream_t);                                           \
     template int launch_ragged_strip<T>(const RaggedLaunch &, const StripPts<T> &, const T *, \

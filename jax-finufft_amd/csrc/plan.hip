@@ -207,6 +207,12 @@ int jfa::setup_strip(jfa_plan p) {
 // One scratch tile per piece of M points, for each transform of a batch.
 static int reserve_strip_scratch(jfa_plan p, int64_t M) {
     const StripGeom &sg = p->sg;
+    if (strip_max_heavy(sg, M) + 1 > p->heavy_cap) {
+        free_ptr(reinterpret_cast<void *&>(p->d_heavy));
+        p->heavy_cap = 0;
+        JFA_HIP_CHECK(hipMalloc(&p->d_heavy, sizeof(int) * (strip_max_heavy(sg, M) + 1)));
+        p->heavy_cap = strip_max_heavy(sg, M) + 1;
+    }
     if (strip_max_pieces(sg, M) > p->piece_unit_cap) {
         free_ptr(reinterpret_cast<void *&>(p->d_piece_unit));
         p->piece_unit_cap = 0;
@@ -801,7 +807,7 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
             if (!tm && ph > 0) break;
             int rc = launch_spread_strip<T>(
                 p->sg, pts, p->d_bin_start, p->d_piece_start, p->d_npieces, p->d_piece_unit,
-                (const T *)p->d_coef,
+                p->d_heavy, (const T *)p->d_coef,
                 kem, p->kp.beta, M, fw + 2 * t0 * fw_stride, fw_stride, (T *)p->d_strip_scratch,
                 p->strip_scratch_stride, nb, tm ? phases[ph] : kStripAll, p->stream);
             if (rc) return rc;
@@ -1040,7 +1046,7 @@ extern "C" int jfa_destroy(jfa_plan p) {
                      (void **)&p->d_val_in, (void **)&p->d_key_out, &p->d_sort_temp, &p->d_t,
                      (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub,
                      (void **)&p->d_piece_start, (void **)&p->d_npieces,
-                     (void **)&p->d_piece_unit, &p->d_strip_scratch,
+                     (void **)&p->d_piece_unit, (void **)&p->d_heavy, &p->d_strip_scratch,
                      &p->d_pack})
         free_ptr(*q);
     jfa::type3_free(p);

@@ -67,6 +67,8 @@ struct jfa_plan_s {
     int *d_piece_start = nullptr, *d_npieces = nullptr;
     int *d_piece_unit = nullptr;  // unit of every piece, strip_max_pieces(sg, M) entries
     int64_t piece_unit_cap = 0;
+    int *d_heavy = nullptr;       // [count, crowded units], 1 + strip_max_heavy(sg, M)
+    int64_t heavy_cap = 0;
     void *d_strip_scratch = nullptr;
     size_t strip_scratch_bytes = 0;
     int64_t strip_scratch_stride = 0;  // complex cells per transform

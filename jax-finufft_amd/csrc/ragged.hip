@@ -39,6 +39,7 @@ struct jfa_ragged_s {
     std::vector<int64_t> moff, mode_blk;
     std::vector<int> vunit_off, unit_virt;
     int64_t max_pieces = 0;  // pieces of every virtual problem at most
+    int64_t max_heavy = 0;   // crowded units (more than one piece) at most
     RaggedLaunch rl{};
 
     // device tables
@@ -57,7 +58,7 @@ struct jfa_ragged_s {
     void *d_sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     int *d_bin_start = nullptr, *d_piece_start = nullptr, *d_npieces = nullptr,
-        *d_piece_unit = nullptr,
+        *d_piece_unit = nullptr, *d_heavy = nullptr,
         *d_blocksum = nullptr;
 
     // fine grids and strip scratch (complex cells)
@@ -113,7 +114,8 @@ void release(jfa_ragged r) {
                      (void **)&r->d_l0, &r->d_t, (void **)&r->d_key_in, (void **)&r->d_val_in,
                      (void **)&r->d_key_out, &r->d_sort_temp, (void **)&r->d_bin_start,
                      (void **)&r->d_piece_start, (void **)&r->d_npieces,
-                     (void **)&r->d_piece_unit, &r->d_fw, &r->d_scratch, &r->d_g,
+                     (void **)&r->d_piece_unit, (void **)&r->d_heavy, &r->d_fw, &r->d_scratch,
+                     &r->d_g,
                      &r->d_fft_work, (void **)&r->d_blocksum})
         free_dev(*q);
     if (r->h_ptrs) (void)hipHostFree((void *)r->h_ptrs);
@@ -224,6 +226,7 @@ int layout(jfa_ragged r) {
     r->vunit_off.assign(1, 0);
     r->unit_virt.clear();
     r->max_pieces = 0;
+    r->max_heavy = 0;
     r->mode_blk.assign(1, 0);
     r->fft.clear();
     int64_t fw = 0, tiles = 0, gc = 0;
@@ -256,6 +259,7 @@ int layout(jfa_ragged r) {
             gc += r->n0[i] * g.nf[1];
             tiles += strip_max_pieces(sg[i], r->npts[i]) * strip_tile_cells(sg[i]);
             r->max_pieces += strip_max_pieces(sg[i], r->npts[i]);
+            r->max_heavy += strip_max_heavy(sg[i], r->npts[i]);
             r->unit_virt.insert(r->unit_virt.end(), sg[i].nunits, (int)r->virt.size() - 1);
             r->vunit_off.push_back(r->vunit_off.back() + sg[i].nunits);
             r->mode_blk.push_back(r->mode_blk.back() + ((v.n0 + 31) / 32) * ((v.n1 + 31) / 32));
@@ -333,6 +337,7 @@ int allocate(jfa_ragged r) {
     JFA_HIP_CHECK(hipMalloc(&r->d_piece_start, sizeof(int) * (r->rl.nunits + 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_npieces, sizeof(int) * 4));
     JFA_HIP_CHECK(hipMalloc(&r->d_piece_unit, sizeof(int) * std::max<int64_t>(r->max_pieces, 1)));
+    JFA_HIP_CHECK(hipMalloc(&r->d_heavy, sizeof(int) * (r->max_heavy + 1)));
     JFA_HIP_CHECK(hipMalloc(&r->d_blocksum, sizeof(int) * (2 * ((r->rl.nunits + 255) / 256) + 1)));
     if (hipMalloc(&r->d_fw, 2 * real * std::max<int64_t>(r->fw_cells, 1)) != hipSuccess ||
         hipMalloc(&r->d_scratch, 2 * real * std::max<int64_t>(r->tile_cells, 1)) != hipSuccess) {
@@ -397,6 +402,12 @@ int allocate(jfa_ragged r) {
     rl.piece_start = r->d_piece_start;
     rl.npieces = r->d_npieces;
     rl.piece_unit = r->d_piece_unit;
+    rl.heavy = r->d_heavy;
+    rl.max_heavy = r->max_heavy;
+    rl.reduce_chunks = 1;
+    for (const RaggedVirt &v : r->virt)
+        rl.reduce_chunks =
+            std::max<int>(rl.reduce_chunks, (int)((strip_tile_cells(v.sg) + 255) / 256));
     rl.blocksum = r->d_blocksum;
     return JFA_OK;
 }
