@@ -264,7 +264,27 @@ __global__ void __launch_bounds__(256)
     ragged_keys_kernel(const RaggedProb *__restrict__ probs, const int64_t *__restrict__ moff,
                        int nprob, const void *const *__restrict__ ptrs, int64_t mtot,
                        unsigned *key, unsigned *val, T *rec) {
-    const int64_t P0 = (int64_t)blockIdx.x * 256 * kRaggedKeysPer + threadIdx.x;
+    const int64_t Pb = (int64_t)blockIdx.x * 256 * kRaggedKeysPer;
+    const int64_t P0 = Pb + threadIdx.x;
+    // most workgroups lie inside one problem: then the geometry is uniform
+    // (scalar loads) and the points independent
+    const int64_t Pe = min(Pb + 256 * kRaggedKeysPer, mtot) - 1;
+    const int ib = problem_of(moff, nprob, Pb);
+    if (Pe < moff[ib + 1]) {
+        const RaggedProb &pr = probs[ib];
+        const T *x = (const T *)ptrs[ib], *y = (const T *)ptrs[nprob + ib];
+#pragma unroll
+        for (int r = 0; r < kRaggedKeysPer; ++r) {
+            const int64_t P = P0 + (int64_t)r * 256;
+            if (P > Pe) break;
+            T tx, ty;
+            key[P] = strip_key<T>(pr.g, pr.bin_off, x, y, P - pr.moff, tx, ty);
+            val[P] = (unsigned)P;
+            rec[2 * P] = tx;
+            rec[2 * P + 1] = ty;
+        }
+        return;
+    }
     if (P0 >= mtot) return;
     int i = problem_of(moff, nprob, P0);
     for (int r = 0; r < kRaggedKeysPer; ++r) {
