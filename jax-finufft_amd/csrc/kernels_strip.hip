@@ -802,7 +802,7 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
                 rl.virt, rl.unit_virt, rl.piece_start, rl.heavy, rl.reduce_chunks, scratch_all);
         }
         if (phases & kStripMerge)
-            ragged_merge_kernel<T, NS_><<<(unsigned)rl.nunits, 256, 0, stream>>>(
+            ragged_merge_kernel<T, NS_><<<(unsigned)rl.nunits, 64, 0, stream>>>(
                 rl.virt, rl.unit_virt, rl.piece_start, scratch_all, fw_all);
     });
     JFA_HIP_CHECK(hipGetLastError());
