@@ -86,7 +86,8 @@ struct StripGeom {
     int64_t nf0, nf1;
     int SW, S;      // strip width (owned first taps) and strip count
     int H, K;       // rows per unit, units per strip
-    int nunits;     // S * K
+    int nz;         // slices (nf2 for a 3D grid, 1 in 2D)
+    int nunits;     // nz * S * K
     int ns;
     int pmax;       // most points per piece (a unit has ceil(count / pmax) pieces)
 };
@@ -232,6 +233,12 @@ template <typename T>
 int launch_sort_points_strip(const GridGeom &g, int64_t M, const T *x, const T *y,
                              const SortScratch &s, int *bin_start, int *idx, T *rec,
                              hipStream_t stream);
+// 3D slices of S strips each: keys of the ns slice points of every point
+// (M * ns of them), sorted; rec3 = (tx, ty, tz) per point.
+template <typename T>
+int launch_sort_points_slices(const GridGeom &g, int S, int64_t M, const T *x, const T *y,
+                              const T *z, const SortScratch &s, int *bin_start, int *idx,
+                              T *rec3, hipStream_t stream);
 
 template <typename T>
 int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const BinIndex &bi,
@@ -240,9 +247,16 @@ int launch_spread(int method, const GridGeom &g, const SortedPts<T> &pts, const 
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, int *piece_unit, int *heavy,
-                        const T *coef, int kerevalmeth, double beta, int64_t pack_stride, T *fw,
+                        int *blocksum, const T *coef, int kerevalmeth, double beta,
+                        int64_t pack_stride, T *fw,
                         int64_t fw_stride, T *scratch, int64_t scratch_stride, int ntr,
                         int phases, hipStream_t stream);
+// 3D slices: pack[t][j * ns + dz] = (c[t][j] * kz_j[dz], tx_j, ty_j) for the ns
+// slice points of point j; rec3[3j .. 3j+2] = (tx, ty, tz) from setpts
+template <typename T>
+int launch_strip3_pack(const T *c, int64_t c_stride, const T *rec3, const T *coef,
+                       int kerevalmeth, double beta, int ns, int64_t M, int ntr, T *pack,
+                       hipStream_t stream);
 // pack[t][j] = (c[t][j], rec[j]) for t < ntr, j < M (records of 4 reals)
 template <typename T>
 int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int ntr, T *pack,

@@ -61,6 +61,33 @@ __device__ __forceinline__ unsigned strip_key(const GridGeom &g, int64_t bin_off
     return (unsigned)((b << kStripKeyShift) | X);
 }
 
+// 3D slices: the ns slice points of point j, keys ((z * S + s) * nf1 + l0y) << 6
+// | X for slices z = l0z + dz (mod nf2), values j * ns + dz; rec3 = (tx, ty, tz).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    slice_keys_kernel(GridGeom g, int S, int64_t M, const T *__restrict__ x,
+                      const T *__restrict__ y, const T *__restrict__ z, unsigned *key,
+                      unsigned *val, T *rec3) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    const Folded f0 = fold_point((double)x[j], g.nf[0], g.scale[0], g.shift[0], g.ns);
+    const Folded f1 = fold_point((double)y[j], g.nf[1], g.scale[1], g.shift[1], g.ns);
+    const Folded f2 = fold_point((double)z[j], g.nf[2], g.scale[2], g.shift[2], g.ns);
+    const int s = f0.l0w / g.bs[0];
+    const int X = f0.l0w - s * g.bs[0];
+    rec3[3 * j] = (T)f0.t;
+    rec3[3 * j + 1] = (T)f1.t;
+    rec3[3 * j + 2] = (T)f2.t;
+    const int nf2 = (int)g.nf[2];
+    for (int dz = 0; dz < g.ns; ++dz) {
+        int zz = f2.l0w + dz;
+        if (zz >= nf2) zz -= nf2;
+        const int64_t b = ((int64_t)zz * S + s) * g.nf[1] + f1.l0w;
+        key[j * g.ns + dz] = (unsigned)((b << kStripKeyShift) | X);
+        val[j * g.ns + dz] = (unsigned)(j * g.ns + dz);
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     strip_keys_kernel(GridGeom g, int64_t M, const T *__restrict__ x, const T *__restrict__ y,
@@ -460,6 +487,19 @@ int launch_sort_points_strip(const GridGeom &g, int64_t M, const T *x, const T *
 }
 
 template <typename T>
+int launch_sort_points_slices(const GridGeom &g, int S, int64_t M, const T *x, const T *y,
+                              const T *z, const SortScratch &s, int *bin_start, int *idx,
+                              T *rec3, hipStream_t stream) {
+    if (M > 0) {
+        const unsigned blocks = (unsigned)((M + 255) / 256);
+        slice_keys_kernel<T><<<blocks, 256, 0, stream>>>(g, S, M, x, y, z, s.key_in, s.val_in,
+                                                         rec3);
+        JFA_HIP_CHECK(hipGetLastError());
+    }
+    return sort_strip_keys(M * g.ns, g.nbins, s, bin_start, idx, stream);
+}
+
+template <typename T>
 int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T *rec,
                        hipStream_t stream) {
     const int64_t M = rl.mtot;
@@ -476,6 +516,9 @@ int launch_ragged_sort(const RaggedLaunch &rl, const SortScratch &s, int *idx, T
 #define JFA_INST(T)                                                                          \
     template int launch_ragged_sort<T>(const RaggedLaunch &, const SortScratch &, int *, T *, \
                                        hipStream_t);                                         \
+    template int launch_sort_points_slices<T>(const GridGeom &, int, int64_t, const T *,      \
+                                              const T *, const T *, const SortScratch &,      \
+                                              int *, int *, T *, hipStream_t);                \
     template int launch_sort_points_strip<T>(const GridGeom &, int64_t, const T *, const T *, \
                                              const SortScratch &, int *, int *, T *,          \
                                              hipStream_t);                                    \

@@ -37,6 +37,14 @@
 // The same per-piece and per-cell bodies serve one plan (a single problem,
 // blockIdx.y = transform) and a ragged batch of problems (descriptor tables,
 // one launch over every problem's units).
+//
+// 3D.  A 3D grid is a stack of nz = nf2 slices of nf0 x nf1 cells.  A point
+// with first taps (l0x, l0y, l0z) and z-weights kz[dz] spreads, in slice
+// l0z + dz, like a 2D point at (l0x, l0y) of strength c * kz[dz]; setpts
+// expands every point into its ns slice points and sorts them by (slice,
+// strip, row), so the strips of all slices form one list of nz * S strips
+// (strip index z * S + s) and the 2D bodies run unchanged.  Only the merge
+// needs the slice: its neighbours are taken within the slice.
 #include "jfa_device.hpp"
 
 namespace jfa {
@@ -408,7 +416,9 @@ __device__ __forceinline__ void merge_unit(const StripGeom &sg, const int *__res
                                            const T *__restrict__ scratch, T *__restrict__ fw,
                                            int u) {
     constexpr int SW = 65 - NS;
-    const int s = u / sg.K, k = u - s * sg.K;
+    const int sz = u / sg.K, k = u - sz * sg.K;  // strip over all slices, unit row
+    const int z = sz / sg.S, s = sz - z * sg.S;  // slice, strip within it
+    fw += 2 * (int64_t)z * sg.nf0 * sg.nf1;
     const int x0 = s * SW, y0 = k * sg.H;
     const int wcols = min(SW, (int)(sg.nf0 - x0));
     const int wrows = min(sg.H, (int)(sg.nf1 - y0));
@@ -429,7 +439,7 @@ __device__ __forceinline__ void merge_unit(const StripGeom &sg, const int *__res
             pfirst[3 * i + j] = 0;
             pcount[3 * i + j] = 0;
             if (i < ncs && j < nck) {
-                const int u2 = cs[i] * sg.K + ck[j];
+                const int u2 = (z * sg.S + cs[i]) * sg.K + ck[j];
                 pfirst[3 * i + j] = piece_start[u2] - base;
                 // a crowded unit's pieces were summed into its first tile
                 pcount[3 * i + j] = min(piece_start[u2 + 1] - piece_start[u2], 1);
@@ -539,6 +549,27 @@ __global__ void __launch_bounds__(256)
     reinterpret_cast<Quad<T> *>(pack + 4 * M * blockIdx.y)[j] = Quad<T>{cv.x, cv.y, tv.x, tv.y};
 }
 
+// 3D: pack[t][j * ns + dz] = (c[t][j] * kz_j[dz], tx_j, ty_j), with the
+// z-weights evaluated here from tz_j (rec3[3j .. 3j+2] = tx, ty, tz).
+template <typename T, int NS>
+__global__ void __launch_bounds__(256)
+    strip3_pack_kernel(const T *__restrict__ c, int64_t c_stride, const T *__restrict__ rec3,
+                       const T *__restrict__ coef, T beta, int horner, int64_t M,
+                       T *__restrict__ pack) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    const Pair<T> cv = reinterpret_cast<const Pair<T> *>(c + 2 * c_stride * blockIdx.y)[j];
+    const T tx = rec3[3 * j], ty = rec3[3 * j + 1], tz = rec3[3 * j + 2];
+    T kz[NS];
+    if (horner)
+        eval_kernel<T, NS, true>(coef, beta, tz, kz);
+    else
+        eval_kernel<T, NS, false>(coef, beta, tz, kz);
+    Quad<T> *out = reinterpret_cast<Quad<T> *>(pack + 4 * NS * M * blockIdx.y) + j * NS;
+#pragma unroll
+    for (int dz = 0; dz < NS; ++dz) out[dz] = Quad<T>{cv.x * kz[dz], cv.y * kz[dz], tx, ty};
+}
+
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
     strip_merge_kernel(StripGeom sg, const int *__restrict__ piece_start,
@@ -548,13 +579,18 @@ __global__ void __launch_bounds__(256)
                       fw + 2 * fw_stride * blockIdx.y, (int)blockIdx.x);
 }
 
+// Piece table of one plan's units in three launches (counts with per-block
+// sums, a scan of the block sums, block-local scans), for the many units of
+// 3D grids; see the ragged form below.
+__global__ void __launch_bounds__(256)
+    strip_units_count_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
+                             int *blocksum);
 __global__ void __launch_bounds__(1024)
-    strip_units_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
-                       int *npieces, int *piece_unit, int *heavy) {
-    piece_scan(
-        sg.nunits, [&](int u) { return unit_pieces(sg, bin_start, u); }, piece_start, npieces,
-        piece_unit, heavy);
-}
+    strip_units_blocks_kernel(int nu, int nblk, const int *__restrict__ blocksum, int *blockoff,
+                              int *piece_start, int *npieces, int *heavy);
+__global__ void __launch_bounds__(256)
+    strip_units_scan_kernel(int nu, const int *__restrict__ blockoff, int *piece_start,
+                            int *piece_unit, int *heavy);
 
 // Sum the tiles of every piece of a crowded unit into its first piece's tile
 // (in piece order), so that the merge reads one tile per unit; work items
@@ -617,6 +653,41 @@ __device__ __forceinline__ int block_scan256(int v, int *total) {
     }
     *total = sa[255];
     return sa[tid] - v;
+}
+
+__global__ void __launch_bounds__(256)
+    strip_units_count_kernel(StripGeom sg, const int *__restrict__ bin_start, int *piece_start,
+                             int *blocksum) {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    const int c = u < sg.nunits ? unit_pieces(sg, bin_start, u) : 0;
+    if (u < sg.nunits) piece_start[u] = c;
+    int total;
+    block_scan256(c, &total);
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024)
+    strip_units_blocks_kernel(int nu, int nblk, const int *__restrict__ blocksum, int *blockoff,
+                              int *piece_start, int *npieces, int *heavy) {
+    piece_scan(
+        nblk, [&](int b) { return blocksum[b]; }, blockoff, npieces, nullptr, nullptr);
+    if (threadIdx.x == 1023) piece_start[nu] = blockoff[nblk];
+    if (threadIdx.x == 0) heavy[0] = 0;  // the scan kernel that follows appends to it
+}
+
+__global__ void __launch_bounds__(256)
+    strip_units_scan_kernel(int nu, const int *__restrict__ blockoff, int *piece_start,
+                            int *piece_unit, int *heavy) {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    const int c = u < nu ? piece_start[u] : 0;
+    int total;
+    const int e = block_scan256(c, &total);
+    if (u < nu) {
+        const int first = blockoff[blockIdx.x] + e;
+        piece_start[u] = first;
+        for (int i = 0; i < c; ++i) piece_unit[first + i] = u;
+        if (c > 1) heavy[1 + atomicAdd(&heavy[0], 1)] = u;
+    }
 }
 
 __global__ void __launch_bounds__(256)
@@ -736,13 +807,19 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *bin_start,
                         int *piece_start, int *npieces, int *piece_unit, int *heavy,
-                        const T *coef, int kerevalmeth, double beta, int64_t pack_stride, T *fw,
+                        int *blocksum, const T *coef, int kerevalmeth, double beta,
+                        int64_t pack_stride, T *fw,
                         int64_t fw_stride, T *scratch, int64_t scratch_stride, int ntr,
                         int phases, hipStream_t stream) {
     if (ntr == 0 || pts.M == 0) return JFA_OK;
     if (phases & kStripUnits) {
-        strip_units_kernel<<<1, 1024, 0, stream>>>(sg, bin_start, piece_start, npieces,
-                                                   piece_unit, heavy);
+        const unsigned nblk = (unsigned)((sg.nunits + 255) / 256);
+        int *blockoff = blocksum + nblk;  // nblk + 1 entries
+        strip_units_count_kernel<<<nblk, 256, 0, stream>>>(sg, bin_start, piece_start, blocksum);
+        strip_units_blocks_kernel<<<1, 1024, 0, stream>>>(sg.nunits, (int)nblk, blocksum,
+                                                          blockoff, piece_start, npieces, heavy);
+        strip_units_scan_kernel<<<nblk, 256, 0, stream>>>(sg.nunits, blockoff, piece_start,
+                                                          piece_unit, heavy);
         JFA_HIP_CHECK(hipGetLastError());
     }
     const int64_t ub = sg.nunits + pts.M / sg.pmax + 1;
@@ -810,6 +887,19 @@ int launch_ragged_strip(const RaggedLaunch &rl, const StripPts<T> &pts, const T 
 }
 
 template <typename T>
+int launch_strip3_pack(const T *c, int64_t c_stride, const T *rec3, const T *coef,
+                       int kerevalmeth, double beta, int ns, int64_t M, int ntr, T *pack,
+                       hipStream_t stream) {
+    if (M == 0 || ntr == 0) return JFA_OK;
+    const dim3 grid((unsigned)((M + 255) / 256), ntr);
+    const T b = (T)beta;
+    JFA_DISPATCH_NS(ns, strip3_pack_kernel<T, NS_><<<grid, 256, 0, stream>>>(
+                            c, c_stride, rec3, coef, b, kerevalmeth, M, pack));
+    JFA_HIP_CHECK(hipGetLastError());
+    return JFA_OK;
+}
+
+template <typename T>
 int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int ntr, T *pack,
                       hipStream_t stream) {
     if (M == 0 || ntr == 0) return JFA_OK;
@@ -821,9 +911,11 @@ int launch_strip_pack(const T *c, int64_t c_stride, const T *rec, int64_t M, int
 
 #define JFA_INST(T)                                                                           \
     template int launch_spread_strip<T>(const StripGeom &, const StripPts<T> &, const int *, \
-                                        int *, int *, int *, int *, const T *, int, double,   \
-                                        int64_t, T *, int64_t, T *, int64_t, int, int,        \
+                                        int *, int *, int *, int *, int *, const T *, int,    \
+                                        double, int64_t, T *, int64_t, T *, int64_t, int, int, \
                                         hipStream_t);                                         \
+    template int launch_strip3_pack<T>(const T *, int64_t, const T *, const T *, int, double, \
+                                       int, int64_t, int, T *, hipStream_t);                  \
     template int launch_strip_pack<T>(const T *, int64_t, const T *, int64_t, int, T *,       \
                                       hipStream_t);                                           \
     template int launch_ragged_strip<T>(const RaggedLaunch &, const StripPts<T> &, const T *, \

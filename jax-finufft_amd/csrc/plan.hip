@@ -94,8 +94,8 @@ int resolve_method(jfa_plan p) {
     if (m == JFA_METHOD_STRIP || (m == JFA_METHOD_AUTO && spreads && strip_applicable(p))) {
         if (spreads) {
             if (!strip_applicable(p)) {
-                set_error("the strip method needs a 2D grid of at least 64 x %d cells",
-                          2 * p->geom.ns);
+                set_error("the strip method needs a 2D or 3D grid of at least 64 x %d (x %d) "
+                          "cells", 2 * p->geom.ns, 2 * p->geom.ns);
                 return -1;
             }
             return JFA_METHOD_STRIP;
@@ -153,39 +153,43 @@ struct DeviceGuard {
 
 bool jfa::strip_applicable(jfa_plan p) {
     const GridGeom &g = p->geom;
-    if (p->dim != 2 || g.nf[0] < 64 || g.nf[1] < 2 * g.ns) return false;
-    // (strip, row) bins shifted by the column bits must fit 32-bit sort keys
+    if (p->dim < 2 || g.nf[0] < 64 || g.nf[1] < 2 * g.ns) return false;
+    if (p->dim == 3 && g.nf[2] < 2 * g.ns) return false;
+    // (slice, strip, row) bins shifted by the column bits must fit 32-bit sort keys
     const int64_t strips = (g.nf[0] + 64 - g.ns) / (64 - g.ns + 1);
-    return ((strips * g.nf[1] + 1) << kStripKeyShift) <= (int64_t)UINT32_MAX;
+    const int64_t nz = p->dim == 3 ? g.nf[2] : 1;
+    return ((nz * strips * g.nf[1] + 1) << kStripKeyShift) <= (int64_t)UINT32_MAX;
 }
 
-// Strip geometry and bins for the 2D strip spreader, and its buffers.  Units
-// of H rows are sized for about 4096 units (a few waves per SIMD), with H in
-// [ns, nf1 - ns + 1] so that a cell is covered by at most two units per axis.
+// Strip geometry and bins for the strip spreader, and its buffers.  Units of
+// H rows are sized for about 4096 units (a few waves per SIMD; a 3D grid's
+// slices give many more, so 16 rows there), with H in [ns, nf1 - ns + 1] so
+// that a cell is covered by at most two units per axis.
 jfa::StripGeom jfa::strip_geometry(GridGeom &g, int pmax, int min_h) {
     StripGeom sg{};
     const int ns = g.ns;
     sg.nf0 = g.nf[0];
     sg.nf1 = g.nf[1];
     sg.ns = ns;
+    sg.nz = g.dim == 3 ? (int)g.nf[2] : 1;
     sg.SW = 64 - ns + 1;
     sg.S = (int)((g.nf[0] + sg.SW - 1) / sg.SW);
-    int64_t H = (g.nf[1] * sg.S + 4095) / 4096;
+    int64_t H = sg.nz > 1 ? 16 : (g.nf[1] * sg.S + 4095) / 4096;
     if (H < min_h) H = min_h;
     if (H > g.nf[1] - ns + 1) H = g.nf[1] - ns + 1;
     if (H < ns) H = ns;
     sg.H = (int)H;
     sg.K = (int)((g.nf[1] + H - 1) / H);
-    sg.nunits = sg.S * sg.K;
+    sg.nunits = sg.nz * sg.S * sg.K;
     sg.pmax = pmax > 0 ? pmax : 4096;
-    // bins: (strip, exact row), rows fastest
+    // bins: (slice, strip, exact row), rows fastest
     g.bs[0] = sg.SW;
     g.bs[1] = 1;
     g.bs[2] = 1;
     g.nb[0] = sg.S;
     g.nb[1] = (int)g.nf[1];
-    g.nb[2] = 1;
-    g.nbins = (int64_t)g.nb[0] * g.nb[1];
+    g.nb[2] = sg.nz;
+    g.nbins = (int64_t)sg.nz * g.nb[0] * g.nb[1];
     g.ymajor = 1;
     // sort keys also carry the first tap's column within the strip
     g.keyshift = kStripKeyShift;
@@ -201,6 +205,9 @@ int jfa::setup_strip(jfa_plan p) {
     free_ptr(reinterpret_cast<void *&>(p->d_piece_start));
     JFA_HIP_CHECK(hipMalloc(&p->d_piece_start, sizeof(int) * (sg.nunits + 1)));
     if (!p->d_npieces) JFA_HIP_CHECK(hipMalloc(&p->d_npieces, sizeof(int) * 4));
+    free_ptr(reinterpret_cast<void *&>(p->d_strip_blocksum));
+    JFA_HIP_CHECK(hipMalloc(&p->d_strip_blocksum,
+                            sizeof(int) * (2 * ((sg.nunits + 255) / 256) + 2)));
     return JFA_OK;
 }
 
@@ -679,6 +686,9 @@ static int sort_points(jfa_plan p, int64_t M, const T *x, const T *y, const T *z
     if (p->method == JFA_METHOD_STRIP) {
         SortScratch s{p->d_key_in, p->d_val_in, p->d_key_out, p->d_sort_temp,
                       p->sort_temp_bytes};
+        if (p->dim == 3)
+            return launch_sort_points_slices<T>(g, p->sg.S, M, x, y, z, s, p->d_bin_start,
+                                                p->d_idx, (T *)p->d_t, p->stream);
         return launch_sort_points_strip<T>(g, M, x, y, s, p->d_bin_start, p->d_idx,
                                            (T *)p->d_t, p->stream);
     }
@@ -709,7 +719,15 @@ int jfa::setpts_12(jfa_plan p, int64_t M, const void *x, const void *y, const vo
             set_error("points for dimension %d are NULL", d);
             return JFA_ERR_BAD_ARG;
         }
-    int rc = reserve_points(p, M);
+    // the 3D strip spreader sorts ns slice points per point
+    const int64_t sorted_count =
+        p->method == JFA_METHOD_STRIP && p->dim == 3 ? M * p->geom.ns : M;
+    if (sorted_count > INT_MAX) {
+        set_error("%lld points make %lld slice points, beyond 2^31 - 1 (use gpu_method 1 or 2)",
+                  (long long)M, (long long)sorted_count);
+        return JFA_ERR_TOO_LARGE;
+    }
+    int rc = reserve_points(p, sorted_count);
     if (rc) return rc;
     p->M = M;
     if (M == 0) return JFA_OK;
@@ -780,10 +798,12 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
     // the merge writes every cell of fw, unless there are no points at all
     const int64_t M = p->M;
     if (M == 0) return launch_fill_zero(fw, (int64_t)sizeof(T) * 2 * fw_stride * ntr, p->stream);
-    if (int rc = reserve_strip_scratch(p, M)) return rc;
+    // points the spreader walks: M, or the ns slice points of each in 3D
+    const int64_t Mq = p->dim == 3 ? M * p->geom.ns : M;
+    if (int rc = reserve_strip_scratch(p, Mq)) return rc;
     // one record (c, tx, ty) per point and transform of a batch: the spreader's
     // only scattered read
-    const size_t pack_bytes = (size_t)std::min(p->batch, ntr) * M * 4 * sizeof(T);
+    const size_t pack_bytes = (size_t)std::min(p->batch, ntr) * Mq * 4 * sizeof(T);
     if (pack_bytes > p->pack_bytes) {
         free_ptr(p->d_pack);
         if (hipMalloc(&p->d_pack, pack_bytes) != hipSuccess) {
@@ -794,12 +814,17 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
         }
         p->pack_bytes = pack_bytes;
     }
-    const StripPts<T> pts{M, p->d_idx, p->d_key_out, (const T *)p->d_pack, nullptr, nullptr, 0, 0};
+    const StripPts<T> pts{Mq, p->d_idx, p->d_key_out, (const T *)p->d_pack, nullptr, nullptr, 0, 0};
     for (int t0 = 0; t0 < ntr; t0 += p->batch) {
         const int nb = std::min(p->batch, ntr - t0);
-        if (int rc = launch_strip_pack<T>(c + 2 * t0 * c_stride, c_stride, (const T *)p->d_t, M,
-                                          nb, (T *)p->d_pack, p->stream))
-            return rc;
+        const int rc =
+            p->dim == 3
+                ? launch_strip3_pack<T>(c + 2 * t0 * c_stride, c_stride, (const T *)p->d_t,
+                                        (const T *)p->d_coef, kem, p->kp.beta, p->geom.ns, M,
+                                        nb, (T *)p->d_pack, p->stream)
+                : launch_strip_pack<T>(c + 2 * t0 * c_stride, c_stride, (const T *)p->d_t, M,
+                                       nb, (T *)p->d_pack, p->stream);
+        if (rc) return rc;
         if (tm) tm->mark("pack");
         const int phases[3] = {kStripUnits, kStripSpread, kStripMerge};
         const char *names[3] = {"units", "spread", "merge"};
@@ -807,8 +832,8 @@ static int spread_timed(jfa_plan p, const T *c, int64_t c_stride, T *fw, int64_t
             if (!tm && ph > 0) break;
             int rc = launch_spread_strip<T>(
                 p->sg, pts, p->d_bin_start, p->d_piece_start, p->d_npieces, p->d_piece_unit,
-                p->d_heavy, (const T *)p->d_coef,
-                kem, p->kp.beta, M, fw + 2 * t0 * fw_stride, fw_stride, (T *)p->d_strip_scratch,
+                p->d_heavy, p->d_strip_blocksum, (const T *)p->d_coef,
+                kem, p->kp.beta, Mq, fw + 2 * t0 * fw_stride, fw_stride, (T *)p->d_strip_scratch,
                 p->strip_scratch_stride, nb, tm ? phases[ph] : kStripAll, p->stream);
             if (rc) return rc;
             if (tm) tm->mark(names[ph]);
@@ -1046,7 +1071,8 @@ extern "C" int jfa_destroy(jfa_plan p) {
                      (void **)&p->d_val_in, (void **)&p->d_key_out, &p->d_sort_temp, &p->d_t,
                      (void **)&p->d_bin_start, (void **)&p->d_sub_start, (void **)&p->d_nsub,
                      (void **)&p->d_piece_start, (void **)&p->d_npieces,
-                     (void **)&p->d_piece_unit, (void **)&p->d_heavy, &p->d_strip_scratch,
+                     (void **)&p->d_piece_unit, (void **)&p->d_heavy,
+                     (void **)&p->d_strip_blocksum, &p->d_strip_scratch,
                      &p->d_pack})
         free_ptr(*q);
     jfa::type3_free(p);

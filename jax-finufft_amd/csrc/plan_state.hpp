@@ -69,6 +69,7 @@ struct jfa_plan_s {
     int64_t piece_unit_cap = 0;
     int *d_heavy = nullptr;       // [count, crowded units], 1 + strip_max_heavy(sg, M)
     int64_t heavy_cap = 0;
+    int *d_strip_blocksum = nullptr;   // piece-scan block sums, 2 ceil(units / 256) + 1
     void *d_strip_scratch = nullptr;
     size_t strip_scratch_bytes = 0;
     int64_t strip_scratch_stride = 0;  // complex cells per transform

@@ -179,6 +179,7 @@ static int type3_setpts_t(jfa_plan p, int64_t M, const T *const xs[3], int64_t N
             int rc = setup_strip(p);
             if (rc) return rc;
             p->method = JFA_METHOD_STRIP;
+            p->method_by_density = false;  // the density choice is between the other two
         }
     }
     // fine grid and prephase buffers

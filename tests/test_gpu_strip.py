@@ -105,3 +105,33 @@ def test_strip_is_bitwise_reproducible(pmax):
     b, _ = _type1_plan((128, 96), x, c, 1e-6, "f32", ntransf=2, gpu_method=3,
                        gpu_maxsubprobsize=pmax)
     assert np.array_equal(a.view(np.float32), b.view(np.float32))
+
+
+@pytest.mark.parametrize("prec,dist,pmax", product(["f32", "f64"], ["uniform", "clustered"],
+                                                   [0, 64]))
+def test_strip_3d_slices_vs_direct_sum(prec, dist, pmax):
+    # 3D spreading as ns 2D slice points per point: every z-slice of the fine
+    # grid a stack of strips (the fastest axis, the last mode count, needs a
+    # fine grid of >= 64 cells)
+    rng = np.random.default_rng(26)
+    M = 3000
+    n_modes = (24, 20, 40)
+    gen = uniform_points if dist == "uniform" else clustered_points
+    x = gen(rng, 3, M, prec)
+    c = cnormal(rng, M, prec)
+    ref = oracle.type1(x, c, n_modes, 1)
+    for eps in EPS[prec][:2]:
+        got, info = _type1_plan(n_modes, x, c, eps, prec, gpu_method=3, gpu_maxsubprobsize=pmax)
+        assert rel_l2(got[0], ref) <= eps, (eps, info, rel_l2(got[0], ref))
+
+
+def test_strip_3d_matches_lds_method_and_is_reproducible():
+    rng = np.random.default_rng(27)
+    M = 20000
+    x = clustered_points(rng, 3, M, "f32")
+    c = cnormal(rng, (2, M), "f32")
+    a, _ = _type1_plan((30, 32, 64), x, c, 1e-6, "f32", ntransf=2, gpu_method=3)
+    b, _ = _type1_plan((30, 32, 64), x, c, 1e-6, "f32", ntransf=2, gpu_method=2)
+    a2, _ = _type1_plan((30, 32, 64), x, c, 1e-6, "f32", ntransf=2, gpu_method=3)
+    assert rel_l2(a, b) < 2e-6
+    assert np.array_equal(a.view(np.float32), a2.view(np.float32))
