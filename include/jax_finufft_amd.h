@@ -57,10 +57,11 @@ enum {
 };
 
 /* Spread/interp method (gpu_method): 0 auto, 1 global-memory atomics in
- * sorted order, 2 LDS subproblem tiles, 3 register strips (2D spreading:
- * atomic-free; interpolation then uses the LDS tiles).  Auto picks 3 for 2D
- * spreading on grids of at least 64 columns, else 2 when the tile fits in
- * LDS, else 1. */
+ * sorted order, 2 LDS subproblem tiles, 3 register strips (2D and 3D
+ * spreading, atomic-free; a 3D grid is spread as a stack of 2D slices;
+ * interpolation then uses the LDS tiles).  Auto picks 3 for spreading on
+ * grids of at least 64 x 2w (x 2w) cells, else 2 when the tile fits in LDS,
+ * else 1. */
 enum { JFA_METHOD_AUTO = 0, JFA_METHOD_GM_SORT = 1, JFA_METHOD_SM = 2, JFA_METHOD_STRIP = 3 };
 
 /* Options.  The first seven fields are the ones the reference forwards to the
