@@ -55,17 +55,17 @@ void free_ptr(void *&p) {
     p = nullptr;
 }
 
-// Bin sizes: 2D 32x32 and 1D 1024 cells; 3D 16x16x8 (single) / 8x8x8
-// (double).  Each is clamped to half the fine grid so that a tile origin plus
-// its extent wraps at most once.
+// Bin sizes: 2D 32x32 and 1D 1024 cells; 3D 8x8x8.  Each is clamped to half
+// the fine grid so that a tile origin plus its extent wraps at most once.  In
+// 3D the smaller tile (14^3 cells at w = 7) keeps more workgroups resident than
+// 16x16x8 (22x22x14): 3D type 2, fp32, measured 4.8 vs 6.3 ms (16.7M koosh-ball
+// points, 256^3 modes), 2.35 vs 2.83 ms and 5.2 vs 6.1 ms (1e7 uniform points,
+// 128^3 / 256^3 modes), 2.7 vs 4.3 ms (2e6 points, 256^3).
 void default_bins(jfa_plan p) {
     int b[3] = {1, 1, 1};
     if (p->dim == 1) b[0] = 1024;
     if (p->dim == 2) b[0] = b[1] = 32;
-    if (p->dim == 3) {
-        if (p->is_double) b[0] = b[1] = b[2] = 8;
-        else { b[0] = b[1] = 16; b[2] = 8; }
-    }
+    if (p->dim == 3) b[0] = b[1] = b[2] = 8;
     const int o[3] = {p->opts.gpu_binsizex, p->opts.gpu_binsizey, p->opts.gpu_binsizez};
     GridGeom &g = p->geom;
     g.nbins = 1;
@@ -600,8 +600,10 @@ int jfa::default_maxsub(jfa_plan p) {
     // default).  Each subproblem sums its points into its tile in fp32/fp64
     // before one global add per cell, so this also bounds the length of the
     // sequential sums a crowded cell sees.  Interpolation accumulates nothing
-    // across points, and larger subproblems load each tile fewer times.
-    return p->type == 2 ? 4096 : 1024;
+    // across points, and larger subproblems load each tile fewer times in 2D;
+    // in 3D, 1024 measured faster (more subproblems in flight: koosh-ball
+    // type 2 on 16.7M points, 4.56 vs 4.80 ms with 8^3 bins).
+    return p->type == 2 && p->dim < 3 ? 4096 : 1024;
 }
 
 // Grow the point-dependent scratch to hold M points.

@@ -155,10 +155,7 @@ static int type3_setpts_t(jfa_plan p, int64_t M, const T *const xs[3], int64_t N
         int bs[3] = {1, 1, 1};
         if (dim == 1) bs[0] = b1;
         if (dim == 2) bs[0] = bs[1] = b2;
-        if (dim == 3) {
-            if (p->is_double) bs[0] = bs[1] = bs[2] = 8;
-            else { bs[0] = bs[1] = 16; bs[2] = 8; }
-        }
+        if (dim == 3) bs[0] = bs[1] = bs[2] = 8;
         g.nbins = 1;
         for (int d = 0; d < 3; ++d) {
             int b = d < dim ? bs[d] : 1;
