@@ -19,10 +19,12 @@ constexpr int kMaxNc = 19;  // most Horner coefficients per kernel interval
 // backend validates.
 constexpr int64_t kMaxFineCells = (int64_t)1 << 32;
 
-// Horner coefficients per interval for width ns: degree ns+4, capped at 18.
-// (Chebyshev-interpolation error at this degree is ~exp(-beta)/2, i.e. below
-// the kernel's own truncation value.)
-constexpr int horner_nc(int ns) { return ns < 14 ? ns + 5 : kMaxNc; }
+// Horner coefficients per interval for width ns: degree ns+2 below ns = 14,
+// else 18.  The Chebyshev-interpolation error is then at most 0.54 exp(-beta)
+// for every width and both upsampling factors, below the kernel's own
+// truncation value; higher degrees gain little (0.46 exp(-beta) at ns+4),
+// since the kernel's square-root edge limits the convergence.
+constexpr int horner_nc(int ns) { return ns < 14 ? ns + 3 : kMaxNc; }
 
 // ------------------------------------------------------------ host math
 

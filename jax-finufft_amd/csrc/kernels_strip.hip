@@ -61,6 +61,13 @@ struct alignas(4 * sizeof(T)) Quad {
     T x, y, z, w;
 };
 
+// Pairs per staged point in the ns <= 8 layout: 8 and one pad.  Each lane
+// writes its own point's pairs, so the point stride is the lane stride of the
+// staging writes; 9 pairs (18 or 36 dwords, twice an odd number) put the
+// lanes of one write on distinct banks, where 8 pairs would put every fourth
+// (64 banks) or second (32 banks) lane on the same bank.
+constexpr int kPointPairs = 9;
+
 // LDS staging of one wave: ns <= 8 uses the weight pairs (and c * y-weights
 // for NT transforms at once), wider kernels the column weights and c *
 // y-weights per point.
@@ -68,13 +75,19 @@ template <typename T, int NS, int NT = 1, bool MOD8 = (NS <= 8)>
 struct StripLds;
 template <typename T, int NS, int NT>
 struct StripLds<T, NS, NT, true> {
-    Pair<T> w[kStripLanes * 8];        // per point q, lane column a: (weight in bx, in bx+1)
-    Pair<T> cky[NT][kStripLanes * 8];  // per transform, point q, row offset dy: c * ky[dy]
+    // per point q, lane column a: (weight in block bx, in bx + 1)
+    Pair<T> w[kStripLanes * kPointPairs];
+    // per transform, point q, row offset dy: c * ky[dy]
+    Pair<T> cky[NT][kStripLanes * kPointPairs];
 };
+// ns > 8: the point stride is rounded up to an odd number of values for the
+// same reason.
+template <int NS>
+constexpr int wide_stride() { return NS | 1; }
 template <typename T, int NS, int NT>
 struct StripLds<T, NS, NT, false> {
-    T kx[kStripLanes * NS];
-    T cky[kStripLanes * NS * 2];
+    T kx[kStripLanes * wide_stride<NS>()];
+    T cky[kStripLanes * wide_stride<NS>() * 2];
     int lx[kStripLanes];
     int ly[kStripLanes];
 };
@@ -116,6 +129,7 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
                                            const int *__restrict__ bin_start, int u, int sub,
                                            const T *__restrict__ coef, T beta, int horner,
                                            T *__restrict__ tile, Lds &lds) {
+    constexpr int NSP = wide_stride<NS>();
     const int lane = threadIdx.x;
     const int rows_tile = sg.H + NS - 1;
     const PieceRange pr = piece_range(sg, bin_start, u, sub);
@@ -158,11 +172,11 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
                 eval_kernel<T, NS, false>(coef, beta, tt.y, ky);
             }
 #pragma unroll
-            for (int a = 0; a < NS; ++a) lds.kx[lane * NS + a] = kx[a];
+            for (int a = 0; a < NS; ++a) lds.kx[lane * NSP + a] = kx[a];
 #pragma unroll
             for (int b = 0; b < NS; ++b) {
-                lds.cky[(lane * NS + b) * 2] = cr * ky[b];
-                lds.cky[(lane * NS + b) * 2 + 1] = ci * ky[b];
+                lds.cky[(lane * NSP + b) * 2] = cr * ky[b];
+                lds.cky[(lane * NSP + b) * 2 + 1] = ci * ky[b];
             }
             lds.lx[lane] = (int)(key & ((1u << kStripKeyShift) - 1));
             lds.ly[lane] = (int)((key >> kStripKeyShift) - rowbase);
@@ -176,8 +190,8 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
             }
             const int kk = lane - lds.lx[q];
             const bool in = kk >= 0 && kk < NS;
-            const T kxv = in ? lds.kx[q * NS + (in ? kk : 0)] : T(0);
-            const T *cy = &lds.cky[q * NS * 2];
+            const T kxv = in ? lds.kx[q * NSP + (in ? kk : 0)] : T(0);
+            const T *cy = &lds.cky[q * NSP * 2];
 #pragma unroll
             for (int b = 0; b < NS; ++b) {
                 acc_r[b] = fma(cy[2 * b], kxv, acc_r[b]);
@@ -196,40 +210,72 @@ __device__ __forceinline__ void wide_piece(const StripGeom &sg, const StripPts<T
 
 // ------------------------------------------------------------ ns <= 8 body
 //
-// Lane (a, r) = (lane & 7, lane >> 3) holds the 8 cells of one row whose
-// columns are a, a+8, ..., a+56 of the strip (accumulator k is column 8k + a),
-// for the window row congruent to r mod 8.  An ns x ns footprint then puts
-// exactly one cell in every lane (ns = 8) and each lane adds one complex
-// product per point instead of ns.
+// Points are staged 64 at a time in wave-private LDS: per point q, its
+// x-weights placed in a 16-column frame starting at the 8-column block
+// bx = X >> 3 of its first tap X (frame column c = (X & 7) + i holds kx[i];
+// stored as pairs by c & 7: (weight at c, weight at c + 8)), and its
+// strength times its y-weights, c * ky[dy].  The wave then walks the batch in
+// runs of equal (row, bx): the sort key's low bits order the points by row and
+// block, so every point of a run adds into the same two column blocks of an
+// 8-row window of 64 columns held in registers (row y in window slot y & 7).
+// When a later row starts, the window's first row is complete: it is written
+// to the piece's tile and its slot cleared.
 //
-// Which of its eight accumulators a lane adds to depends on the point's
-// 8-column block bx = X >> 3 (X = first tap within the strip) and on whether
-// the lane's column a lies before the point's phase X & 7 (then it is the next
-// block).  The first is made wave-uniform by sorting the points by (row, bx)
-// (sort key low bits) and walking them in runs of equal (row, bx) under a
-// switch; the second by staging, per point and lane column, the weight split
-// into a pair (weight in block bx, weight in block bx + 1), one of them zero.
+// Lane (a, r) = (lane & 7, lane >> 3) holds columns a, a+8, ..., a+56 of the
+// row in slot r.  An ns x ns footprint (ns <= 8) puts at most one cell in
+// every lane, so per point each lane adds the products of its weight pair
+// with the point's c * ky for its row into blocks bx and bx + 1.
 
-template <typename T, int BX, int NT>
-__device__ __forceinline__ void strip8_walk(const Pair<T> *__restrict__ s_w,
-                                            const Pair<T> (*__restrict__ s_cky)[kStripLanes * 8],
-                                            int q0, int q1, int a, int dy, T (&acc_r)[NT][8],
-                                            T (&acc_i)[NT][8]) {
-#pragma unroll 4
-    for (int q = q0; q < q1; ++q) {
-        const Pair<T> w = s_w[q * 8 + a];
+// The accumulator window (see above).
+template <typename T, int NT>
+struct Window8 {
+    T re[NT][8], im[NT][8];
+
+    __device__ __forceinline__ void clear() {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const Pair<T> cy = s_cky[t][q * 8 + dy];
-            acc_r[t][BX] = fma(w.x, cy.x, acc_r[t][BX]);
-            acc_i[t][BX] = fma(w.x, cy.y, acc_i[t][BX]);
-            if constexpr (BX < 7) {  // block 8 would only receive zero weights
-                acc_r[t][BX + 1] = fma(w.y, cy.x, acc_r[t][BX + 1]);
-                acc_i[t][BX + 1] = fma(w.y, cy.y, acc_i[t][BX + 1]);
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) re[t][b] = im[t][b] = T(0);
+    }
+
+    // points [q0, q1) of the staged batch, all in row Y and block BX
+    template <int BX>
+    __device__ __forceinline__ void walk(const Pair<T> *__restrict__ s_w,
+                                         const Pair<T> (*__restrict__ s_cky)
+                                             [kStripLanes * kPointPairs],
+                                         int q0, int q1, int Y, int lane) {
+        const int a = lane & 7, dy = ((lane >> 3) - Y) & 7;
+#pragma unroll 4
+        for (int q = q0; q < q1; ++q) {
+            const Pair<T> w = s_w[q * kPointPairs + a];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const Pair<T> cy = s_cky[t][q * kPointPairs + dy];
+                re[t][BX] = fma(w.x, cy.x, re[t][BX]);
+                im[t][BX] = fma(w.x, cy.y, im[t][BX]);
+                if constexpr (BX < 7) {  // block 8 would only receive zero weights
+                    re[t][BX + 1] = fma(w.y, cy.x, re[t][BX + 1]);
+                    im[t][BX + 1] = fma(w.y, cy.y, im[t][BX + 1]);
+                }
             }
         }
     }
-}
+
+    // write window row `row` (tile row row - r0) and clear its slot
+    __device__ __forceinline__ void flush(int row, int r0, int lane, T *__restrict__ tile,
+                                          int64_t tile_tstride) {
+        if ((row & 7) != (lane >> 3)) return;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile + t * tile_tstride) +
+                           (int64_t)(row - r0) * kStripLanes + (lane & 7);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) dst[8 * b] = Pair<T>{re[t][b], im[t][b]};
+#pragma unroll
+            for (int b = 0; b < 8; ++b) re[t][b] = im[t][b] = T(0);
+        }
+    }
+};
 
 // One piece for NT transforms sharing the points: transform k's strengths at
 // pts.c + k * pts.c_tstride (NT = 1 may read packed records instead), its tile
@@ -243,41 +289,58 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
     static_assert(NS <= 8, "modular ownership needs ns <= 8");
     static_assert(NT <= LNT, "LDS staging for NT transforms");
     Pair<T> *s_w = lds.w;
-    Pair<T> (*s_cky)[kStripLanes * 8] = lds.cky;
+    Pair<T> (*s_cky)[kStripLanes * kPointPairs] = lds.cky;
     const int lane = threadIdx.x;
-    const int a = lane & 7, r = lane >> 3;
     const int rows_tile = sg.H + NS - 1;
     const PieceRange pr = piece_range(sg, bin_start, u, sub);
     const int r0 = pr.r0, pb = pr.pb, pe = pr.pe;
     // key >> kStripKeyShift = rowbase + l0y for every point of this strip
     const int64_t rowbase = pts.bin_off + (int64_t)(u / sg.K) * sg.nf1;
-    T acc_r[NT][8], acc_i[NT][8];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int b = 0; b < 8; ++b) acc_r[t][b] = acc_i[t][b] = T(0);
-    // the window holds rows R .. R+7; row y sits in the lanes with r == y & 7
+    Window8<T, NT> win;
+    win.clear();
+    // the window holds rows R .. R+7
     int R = r0;
-    auto flush = [&](int row) {
-        if ((row & 7) == r) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                Pair<T> *dst = reinterpret_cast<Pair<T> *>(tile + t * tile_tstride) +
-                               (int64_t)(row - r0) * kStripLanes + a;
-#pragma unroll
-                for (int b = 0; b < 8; ++b) dst[8 * b] = Pair<T>{acc_r[t][b], acc_i[t][b]};
-#pragma unroll
-                for (int b = 0; b < 8; ++b) acc_r[t][b] = acc_i[t][b] = T(0);
-            }
+    // Software pipeline over 64-point batches: while one batch is staged and
+    // walked, the next batch's records (and other transforms' strengths) and
+    // the key and index of the batch after it are in flight.  A wave holds one
+    // unit, so without this every batch would wait for its scattered record
+    // reads with too few other waves on the SIMD to cover them.
+    constexpr int NC = NT > 1 ? NT - 1 : 1;
+    unsigned key1 = 0, key2 = 0;
+    int idx1 = 0, idx2 = 0;
+    Quad<T> rq1{};
+    Pair<T> ct1[NC];
+    auto load_index = [&](int b, unsigned &k, int &i) {
+        if (b + lane < pe) {
+            k = pts.key[b + lane];
+            i = pts.idx[b + lane];
         }
     };
+    auto load_record = [&](int b, int i) {
+        if (b + lane < pe) {
+            rq1 = point_record(pts, i);
+#pragma unroll
+            for (int t = 1; t < NT; ++t)
+                ct1[t - 1] = reinterpret_cast<const Pair<T> *>(
+                    pts.c + t * pts.c_tstride)[i - pts.idx_off];
+        }
+    };
+    load_index(pb, key1, idx1);
+    load_record(pb, idx1);
+    load_index(pb + kStripLanes, key2, idx2);
     for (int bb = pb; bb < pe; bb += kStripLanes) {
         const int nb = min(kStripLanes, pe - bb);
+        const unsigned pkey = key1;
+        const Quad<T> rq = rq1;
+        Pair<T> ct[NC];
+#pragma unroll
+        for (int t = 0; t < NC; ++t) ct[t] = ct1[t];
+        key1 = key2;
+        idx1 = idx2;
+        load_record(bb + kStripLanes, idx1);
+        load_index(bb + 2 * kStripLanes, key2, idx2);
         int key = -1;  // (row << 3) | block of the lane's staged point
         if (lane < nb) {
-            const int p = bb + lane;
-            const unsigned pkey = pts.key[p];
-            const Quad<T> rq = point_record(pts, pts.idx[p]);
             const T cr = rq.x, ci = rq.y;
             const Pair<T> tt{rq.z, rq.w};
             const int X = (int)(pkey & ((1u << kStripKeyShift) - 1));
@@ -293,7 +356,8 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
                 for (int i = 0; i < 8; ++i) {
                     const T v = i < NS ? kx[i] : T(0);
                     const bool lo = ph + i < 8;
-                    s_w[lane * 8 + ((ph + i) & 7)] = Pair<T>{lo ? v : T(0), lo ? T(0) : v};
+                    s_w[lane * kPointPairs + ((ph + i) & 7)] =
+                        Pair<T>{lo ? v : T(0), lo ? T(0) : v};
                 }
             }
             {
@@ -304,16 +368,15 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
                     eval_kernel<T, NS, false>(coef, beta, tt.y, ky);
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
-                    s_cky[0][lane * 8 + i] =
+                    s_cky[0][lane * kPointPairs + i] =
                         i < NS ? Pair<T>{cr * ky[i], ci * ky[i]} : Pair<T>{T(0), T(0)};
 #pragma unroll
                 for (int t = 1; t < NT; ++t) {
-                    const Pair<T> ct = reinterpret_cast<const Pair<T> *>(
-                        pts.c + t * pts.c_tstride)[pts.idx[p] - pts.idx_off];
 #pragma unroll
                     for (int i = 0; i < 8; ++i)
-                        s_cky[t][lane * 8 + i] = i < NS ? Pair<T>{ct.x * ky[i], ct.y * ky[i]}
-                                                        : Pair<T>{T(0), T(0)};
+                        s_cky[t][lane * kPointPairs + i] =
+                            i < NS ? Pair<T>{ct[t - 1].x * ky[i], ct[t - 1].y * ky[i]}
+                                   : Pair<T>{T(0), T(0)};
                 }
             }
             key = ((int)((pkey >> kStripKeyShift) - rowbase) << 3) | (X >> 3);
@@ -329,19 +392,18 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
             const int gkey = __builtin_amdgcn_readlane(key, q0);
             const int Y = gkey >> 3;
             while (R < Y) {
-                flush(R);
+                win.flush(R, r0, lane, tile, tile_tstride);
                 ++R;
             }
-            const int dy = (r - Y) & 7;
             switch (gkey & 7) {
-                case 0: strip8_walk<T, 0, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 1: strip8_walk<T, 1, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 2: strip8_walk<T, 2, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 3: strip8_walk<T, 3, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 4: strip8_walk<T, 4, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 5: strip8_walk<T, 5, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                case 6: strip8_walk<T, 6, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
-                default: strip8_walk<T, 7, NT>(s_w, s_cky, q0, q1, a, dy, acc_r, acc_i); break;
+                case 0: win.template walk<0>(s_w, s_cky, q0, q1, Y, lane); break;
+                case 1: win.template walk<1>(s_w, s_cky, q0, q1, Y, lane); break;
+                case 2: win.template walk<2>(s_w, s_cky, q0, q1, Y, lane); break;
+                case 3: win.template walk<3>(s_w, s_cky, q0, q1, Y, lane); break;
+                case 4: win.template walk<4>(s_w, s_cky, q0, q1, Y, lane); break;
+                case 5: win.template walk<5>(s_w, s_cky, q0, q1, Y, lane); break;
+                case 6: win.template walk<6>(s_w, s_cky, q0, q1, Y, lane); break;
+                default: win.template walk<7>(s_w, s_cky, q0, q1, Y, lane); break;
             }
         }
         __syncthreads();
@@ -349,7 +411,7 @@ __device__ __forceinline__ void strip8_piece(const StripGeom &sg, const StripPts
     // drain the window and every remaining row of the tile
     const int rstop = r0 + rows_tile;
     while (R < rstop) {
-        flush(R);
+        win.flush(R, r0, lane, tile, tile_tstride);
         ++R;
     }
 }
