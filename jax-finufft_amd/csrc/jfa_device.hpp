@@ -70,30 +70,6 @@ __device__ __forceinline__ void eval_kernel(const T *__restrict__ coef, T beta, 
     }
 }
 
-// Kernel values and their derivatives d/dr (per cell of offset) for one point
-// along one dimension: the derivative of the same fitted polynomial, times
-// dt/dr = 2.
-template <typename T, int NS>
-__device__ __forceinline__ void eval_kernel_deriv(const T *__restrict__ coef, T t, T *ker,
-                                                  T *dker) {
-    constexpr int NC = horner_nc(NS);
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        ker[i] = coef[(NC - 1) * NS + i];
-        dker[i] = T(0);
-    }
-#pragma unroll
-    for (int p = NC - 2; p >= 0; --p) {
-#pragma unroll
-        for (int i = 0; i < NS; ++i) {
-            dker[i] = fma(dker[i], t, ker[i]);
-            ker[i] = fma(ker[i], t, coef[p * NS + i]);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < NS; ++i) dker[i] *= T(2);
-}
-
 __device__ __forceinline__ void atomic_add_global(float *p, float v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add_global(double *p, double v) { unsafeAtomicAdd(p, v); }
 
