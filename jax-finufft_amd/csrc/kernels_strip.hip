@@ -884,8 +884,10 @@ int launch_spread_strip(const StripGeom &sg, const StripPts<T> &pts, const int *
                                                           piece_unit, heavy);
         JFA_HIP_CHECK(hipGetLastError());
     }
+    // one workgroup per piece (see launch_ragged_strip); 3D slice grids have
+    // tens of thousands of units
     const int64_t ub = sg.nunits + pts.M / sg.pmax + 1;
-    const unsigned grid = (unsigned)(ub < 256 * 32 ? ub : 256 * 32);
+    const unsigned grid = (unsigned)(ub < (1 << 30) ? ub : (1 << 30));
     const T b = (T)beta;
     JFA_DISPATCH_NS(sg.ns, {
         if (phases & kStripSpread)
